@@ -1,0 +1,639 @@
+// hana_kernels.hip -- the gfx950 rasterisation pipeline.
+//
+// One launch sequence renders any number of passes (a pass is one
+// graphics_draw_triangle call, SR/graphics.cpp:378-407):
+//
+//   k_vertex   one vertex per lane: vertex program, clip-space trivial-accept
+//              bit, perspective divide and 1/w          (SR/IShader.cpp, SR/graphics.cpp:132-134, 316-337)
+//   k_setup<0> one face per lane: clipping of faces that fail the trivial
+//              accept (SR/graphics.cpp:136-161), back-face cull, viewport,
+//              bbox, degenerate test; counts (triangle, tile) pairs
+//   k_scan     exclusive scan of the per-tile counts
+//   k_setup<1> same set-up; scatters triangle ids into the per-tile bins
+//   k_raster   one workgroup per 32x32 tile: exact per-pixel coverage and
+//              depth (SR/graphics.cpp:222-233, 349-359) resolved in LDS with
+//              64-bit (depth, submission order) keys, then one fragment
+//              program per covered pixel (SR/graphics.cpp:362-373)
+//
+// Depth test semantics.  The reference tests `z > stored` per fragment in
+// submission order (LEQUAL, later primitive wins ties) and no shader
+// discards, so the final pixel is the last fragment in submission order among
+// those at the minimum depth.  Each fragment here forms the key
+// (order-preserving bits of z, 0xFFFFFFFE - order) and the tile keeps the
+// per-pixel minimum with ds_min_u64: the minimum is that same fragment
+// whatever order the lanes run in.  NaN depths (which pass any test and make
+// the next fragment pass) are resolved exactly by a second sweep.
+#include <float.h>
+#include <hip/hip_runtime.h>
+
+#include "hana_device.h"
+#include "hana_launch.h"
+
+namespace hana {
+
+// ---------------------------------------------------------------------------
+// shared helpers
+// ---------------------------------------------------------------------------
+
+template <int WHICH>
+__device__ __forceinline__ int block0_of(const PassDesc& p) {
+    return WHICH == 0 ? p.vblock0 : (WHICH == 1 ? p.fblock0 : p.tblock0);
+}
+// largest pass index whose first block is <= block (passes sorted by it)
+template <int WHICH>
+__device__ __forceinline__ int find_pass(const PassDesc* P, int n, int block) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (block0_of<WHICH>(P[mid]) <= block) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ bool vis_bit(const uint64_t* vis, int i) { return (vis[i >> 6] >> (i & 63)) & 1ull; }
+
+// Triangle set-up in screen space (SR/graphics.cpp:316-347, SR/maths.cpp:20-25).
+struct TriS {
+    float ax, ay;        // screen A
+    float s0x, s0y;      // C.x - A.x, B.x - A.x
+    float s1x, s1y;      // C.y - A.y, B.y - A.y
+    float uz;            // cross product z (constant over the triangle)
+    float z0, z1, z2;    // screen depths
+    int x0, x1, y0, y1;  // pixel loop bounds (inclusive)
+};
+
+// Returns false for triangles that can produce no fragment: back-facing
+// (is_back_facing, SR/graphics.cpp:172-180), degenerate (|u.z| <= 1e-2, every
+// barycentric then (-1,1,1), SR/graphics.cpp:230-232) or with an empty scan.
+__device__ __forceinline__ bool tri_setup(float4 a, float4 b, float4 c, int W, int H, TriS& t) {
+    float signed_area = a.x * b.y - a.y * b.x + b.x * c.y - b.y * c.x + c.x * a.y - c.y * a.x;
+    if (signed_area <= 0) return false;
+    float fw = (float)W, fh = (float)H;
+    float sx0 = (a.x + 1) * 0.5f * fw, sy0 = (a.y + 1) * 0.5f * fh;
+    float sx1 = (b.x + 1) * 0.5f * fw, sy1 = (b.y + 1) * 0.5f * fh;
+    float sx2 = (c.x + 1) * 0.5f * fw, sy2 = (c.y + 1) * 0.5f * fh;
+    t.z0 = (a.z + 1) * 0.5f;
+    t.z1 = (b.z + 1) * 0.5f;
+    t.z2 = (c.z + 1) * 0.5f;
+    t.ax = sx0;
+    t.ay = sy0;
+    t.s0x = sx2 - sx0;
+    t.s0y = sx1 - sx0;
+    t.s1x = sy2 - sy0;
+    t.s1y = sy1 - sy0;
+    t.uz = t.s0x * t.s1y - t.s0y * t.s1x;
+    // (double)|u.z| > 1e-2 is the same test as |u.z| > 0.01f on floats
+    if (!(fabsf(t.uz) > 0.01f)) return false;
+    float mnx = FLT_MAX, mny = FLT_MAX, mxx = -FLT_MAX, mxy = -FLT_MAX;
+    float clx = (float)(W - 1), cly = (float)(H - 1);
+    mnx = smax(0.f, smin(mnx, sx0)); mxx = smin(clx, smax(mxx, sx0));
+    mny = smax(0.f, smin(mny, sy0)); mxy = smin(cly, smax(mxy, sy0));
+    mnx = smax(0.f, smin(mnx, sx1)); mxx = smin(clx, smax(mxx, sx1));
+    mny = smax(0.f, smin(mny, sy1)); mxy = smin(cly, smax(mxy, sy1));
+    mnx = smax(0.f, smin(mnx, sx2)); mxx = smin(clx, smax(mxx, sx2));
+    mny = smax(0.f, smin(mny, sy2)); mxy = smin(cly, smax(mxy, sy2));
+    // for (P.x = bboxmin.x; P.x <= bboxmax.x; P.x++): start truncates, the
+    // bound compares the int as a float
+    t.x0 = f2i(mnx);
+    t.y0 = f2i(mny);
+    if (t.x0 < 0 || t.y0 < 0) return false;
+    t.x1 = (mxx >= 0.f) ? (int)floorf(mxx) : -1;
+    t.y1 = (mxy >= 0.f) ? (int)floorf(mxy) : -1;
+    return t.x1 >= t.x0 && t.y1 >= t.y0;
+}
+
+// barycentric + coverage + interpolate_depth at integer pixel (px, py)
+// (SR/graphics.cpp:186-194, 222-233, 352-356).
+__device__ __forceinline__ bool tri_cover(const TriS& t, int px, int py, float& wx, float& wy, float& wz, float& z) {
+    float Px = (float)px, Py = (float)py;
+    float s0z = t.ax - Px;
+    float s1z = t.ay - Py;
+    float ux = t.s0y * s1z - s0z * t.s1y;
+    float uy = s0z * t.s1x - t.s0x * s1z;
+    wx = 1.f - (ux + uy) / t.uz;
+    wy = uy / t.uz;
+    wz = ux / t.uz;
+    if (wx < 0 || wy < 0 || wz < 0) return false;
+    z = 0.0f;
+    z += t.z2 * wz;
+    z += t.z1 * wy;
+    z += t.z0 * wx;
+    return true;
+}
+
+// order-preserving 32-bit image of a depth (-0 folded onto +0)
+__device__ __forceinline__ uint32_t depth_bits(float z) {
+    uint32_t u = __float_as_uint(z);
+    if ((u & 0x7FFFFFFFu) == 0) u = 0;
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ uint64_t init_key(float d) {
+    uint32_t hi = (d != d) ? 0xFFFFFFFFu : depth_bits(d);   // NaN: the first fragment passes
+    return ((uint64_t)hi << 32) | 0xFFFFFFFFull;
+}
+
+__device__ __forceinline__ void stat_add(const PassDesc& P, int which, uint64_t v) {
+    if (P.stats && v) atomicAdd((unsigned long long*)&P.stats[which], (unsigned long long)v);
+}
+
+// ---------------------------------------------------------------------------
+// k_vertex
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(BLOCK) void k_vertex(const PassDesc* passes, int npasses, Workspace ws) {
+    int gtid = blockIdx.x * BLOCK + threadIdx.x;
+    int gsize = gridDim.x * BLOCK;
+    for (int i = gtid; i < ws.ntiles; i += gsize) ws.tile_count[i] = 0;
+    if (gtid < STATUS_WORDS) ws.status[gtid] = 0;
+
+    const PassDesc& P = passes[find_pass<0>(passes, npasses, blockIdx.x)];
+    int v = (blockIdx.x - P.vblock0) * BLOCK + threadIdx.x;
+    bool vis = false;
+    if (v < P.nverts) {
+        const DevUniforms& u = *P.u;
+        F3 p = f3(P.px[v], P.py[v], P.pz[v]);
+        bool need_n = P.kind != SHADER_SHADOW && P.kind != SHADER_TEXTURE;
+        F3 n = need_n ? f3(P.nx[v], P.ny[v], P.nz[v]) : f3(0.f, 0.f, 0.f);
+        float tu = P.tu[v], tv = P.tv[v];
+        float clip[4], vary[8];
+        vertex_program(P.kind, u, p, n, tu, tv, clip, vary);
+        // is_vertex_visible (SR/graphics.cpp:132-134)
+        vis = fabsf(clip[0]) <= clip[3] && fabsf(clip[1]) <= clip[3] && fabsf(clip[2]) <= clip[3];
+        // clip_pos / clip_pos[3] and 1 / clip_pos[3] (SR/graphics.cpp:317, 336)
+        float4 r;
+        r.w = 1 / clip[3];
+        r.z = clip[2] / clip[3];
+        r.y = clip[1] / clip[3];
+        r.x = clip[0] / clip[3];
+        P.vr[v] = r;
+        if (P.nvary == 8) {
+            float4* d = reinterpret_cast<float4*>(P.vs + (size_t)v * 8);
+            d[0] = make_float4(vary[0], vary[1], vary[2], vary[3]);
+            d[1] = make_float4(vary[4], vary[5], vary[6], vary[7]);
+        } else if (P.nvary == 2) {
+            reinterpret_cast<float2*>(P.vs)[v] = make_float2(vary[0], vary[1]);
+        } else {
+            P.vs[v] = vary[0];
+        }
+    }
+    uint64_t bits = __ballot(vis);
+    int wave_first = (blockIdx.x - P.vblock0) * BLOCK + (threadIdx.x & ~63);
+    if ((threadIdx.x & 63) == 0 && wave_first < P.nverts) P.vis[wave_first >> 6] = bits;
+}
+
+// ---------------------------------------------------------------------------
+// clipping (SR/graphics.cpp:7-161), for faces failing the trivial accept
+// ---------------------------------------------------------------------------
+
+enum { PL_W, PL_PX, PL_NX, PL_PY, PL_NY, PL_PZ, PL_NZ };
+
+__device__ __forceinline__ bool inside_plane(const float* c, int plane) {
+    switch (plane) {
+    case PL_W: return c[3] >= 1e-5f;
+    case PL_PX: return c[0] <= +c[3];
+    case PL_NX: return c[0] >= -c[3];
+    case PL_PY: return c[1] <= +c[3];
+    case PL_NY: return c[1] >= -c[3];
+    case PL_PZ: return c[2] <= +c[3];
+    default: return c[2] >= -c[3];
+    }
+}
+__device__ __forceinline__ float intersect_ratio(const float* p, const float* c, int plane) {
+    switch (plane) {
+    case PL_W: return (p[3] - 1e-5f) / (p[3] - c[3]);
+    case PL_PX: return (p[3] - p[0]) / ((p[3] - p[0]) - (c[3] - c[0]));
+    case PL_NX: return (p[3] + p[0]) / ((p[3] + p[0]) - (c[3] + c[0]));
+    case PL_PY: return (p[3] - p[1]) / ((p[3] - p[1]) - (c[3] - c[1]));
+    case PL_NY: return (p[3] + p[1]) / ((p[3] + p[1]) - (c[3] + c[1]));
+    case PL_PZ: return (p[3] - p[2]) / ((p[3] - p[2]) - (c[3] - c[2]));
+    default: return (p[3] + p[2]) / ((p[3] + p[2]) - (c[3] + c[2]));
+    }
+}
+
+// One Sutherland-Hodgman stage over vertices of NF floats (clip pos first);
+// every float is interpolated as liner_interpolate_varyings does.  A convex
+// polygon gains at most one vertex per plane (3 + 7 = POLY_MAX); an output
+// that would exceed POLY_MAX (only possible through rounding) returns -1 and
+// the polygon is dropped and flagged.
+template <int NF>
+__device__ __forceinline__ int clip_stage(int plane, int n, const float (*in)[NF], float (*out)[NF]) {
+    int m = 0;
+    for (int i = 0; i < n; i++) {
+        int prev = (i - 1 + n) % n;
+        bool pin = inside_plane(in[prev], plane), cin = inside_plane(in[i], plane);
+        if (m + (pin != cin) + (cin ? 1 : 0) > POLY_MAX) return -1;
+        if (pin != cin) {
+            float t = intersect_ratio(in[prev], in[i], plane);
+            for (int k = 0; k < NF; k++) out[m][k] = in[prev][k] + (in[i][k] - in[prev][k]) * t;
+            m++;
+        }
+        if (cin) {
+            for (int k = 0; k < NF; k++) out[m][k] = in[i][k];
+            m++;
+        }
+    }
+    return m;
+}
+// Returns the vertex count (result in b), 0 when fewer than 3 remain, -1 on
+// a polygon beyond POLY_MAX vertices.
+template <int NF>
+__device__ __forceinline__ int clip_polygon(float (*a)[NF], float (*b)[NF]) {
+    int n = 3;
+    n = clip_stage<NF>(PL_W, n, a, b); if (n < 3) return n < 0 ? -1 : 0;
+    n = clip_stage<NF>(PL_PX, n, b, a); if (n < 3) return n < 0 ? -1 : 0;
+    n = clip_stage<NF>(PL_NX, n, a, b); if (n < 3) return n < 0 ? -1 : 0;
+    n = clip_stage<NF>(PL_PY, n, b, a); if (n < 3) return n < 0 ? -1 : 0;
+    n = clip_stage<NF>(PL_NY, n, a, b); if (n < 3) return n < 0 ? -1 : 0;
+    n = clip_stage<NF>(PL_PZ, n, b, a); if (n < 3) return n < 0 ? -1 : 0;
+    n = clip_stage<NF>(PL_NZ, n, a, b); if (n < 3) return n < 0 ? -1 : 0;
+    return n;
+}
+
+// Re-runs the vertex program on the face's corners, clips, and stores the
+// polygon (perspective-divided) in the pool.  Returns the polygon id or -1.
+__device__ __noinline__ int clip_face(const PassDesc& P, const Workspace& ws, int f, int i0, int i1, int i2) {
+    const DevUniforms& u = *P.u;
+    float a[POLY_MAX][4 + POLY_VARY], b[POLY_MAX][4 + POLY_VARY];
+    int ids[3] = {i0, i1, i2};
+    bool need_n = P.kind != SHADER_SHADOW && P.kind != SHADER_TEXTURE;
+    for (int k = 0; k < 3; k++) {
+        int v = ids[k];
+        F3 p = f3(P.px[v], P.py[v], P.pz[v]);
+        F3 n = need_n ? f3(P.nx[v], P.ny[v], P.nz[v]) : f3(0.f, 0.f, 0.f);
+        vertex_program(P.kind, u, p, n, P.tu[v], P.tv[v], a[k], a[k] + 4);
+    }
+    // positions only first: most such faces lie wholly outside and are dropped
+    {
+        float pa[POLY_MAX][4], pb[POLY_MAX][4];
+        for (int k = 0; k < 3; k++)
+            for (int c = 0; c < 4; c++) pa[k][c] = a[k][c];
+        int np = clip_polygon<4>(pa, pb);
+        if (np < 0) atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_POLY_TOO_LARGE);
+        if (np < 3) return -1;
+    }
+    int n = clip_polygon<4 + POLY_VARY>(a, b);
+    if (n < 3) return -1;
+    int pid = (int)atomicAdd(&ws.status[STATUS_POLYS], 1u);
+    if ((uint32_t)pid >= ws.poly_cap) {
+        atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_POLY_OVERFLOW);
+        return -1;
+    }
+    for (int k = 0; k < n; k++) {
+        float4 r;
+        r.w = 1 / b[k][3];
+        r.z = b[k][2] / b[k][3];
+        r.y = b[k][1] / b[k][3];
+        r.x = b[k][0] / b[k][3];
+        ws.pool_vr[(size_t)pid * POLY_MAX + k] = r;
+        for (int c = 0; c < POLY_VARY; c++) ws.pool_vs[((size_t)pid * POLY_MAX + k) * POLY_VARY + c] = b[k][4 + c];
+    }
+    ws.pool_face[pid] = f;
+    ws.pool_n[pid] = n;
+    return pid;
+}
+
+// ---------------------------------------------------------------------------
+// k_setup: count (SCATTER = 0) or scatter (SCATTER = 1) (triangle, tile) pairs
+// ---------------------------------------------------------------------------
+
+template <int SCATTER>
+__device__ __forceinline__ void emit(const PassDesc& P, const Workspace& ws, const TriS& t, uint32_t entry) {
+    int tx0 = t.x0 >> TILE_SHIFT, tx1 = t.x1 >> TILE_SHIFT;
+    int ty0 = t.y0 >> TILE_SHIFT, ty1 = t.y1 >> TILE_SHIFT;
+    for (int ty = ty0; ty <= ty1; ty++)
+        for (int tx = tx0; tx <= tx1; tx++) {
+            int gt = P.tile0 + ty * P.tiles_x + tx;
+            if (!SCATTER) {
+                atomicAdd(&ws.tile_count[gt], 1u);
+            } else {
+                uint32_t pos = atomicAdd(&ws.tile_cursor[gt], 1u);
+                if (pos < ws.bin_cap) ws.bins[pos] = entry;
+            }
+        }
+    if (!SCATTER) stat_add(P, ST_ENTRIES, (uint64_t)(tx1 - tx0 + 1) * (uint64_t)(ty1 - ty0 + 1));
+}
+
+template <int SCATTER>
+__global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* passes, int npasses, Workspace ws) {
+    const PassDesc& P = passes[find_pass<1>(passes, npasses, blockIdx.x)];
+    int f = (blockIdx.x - P.fblock0) * BLOCK + threadIdx.x;
+    if (f >= P.nfaces) return;
+    int i0 = P.idx[3 * f], i1 = P.idx[3 * f + 1], i2 = P.idx[3 * f + 2];
+    bool trivial = vis_bit(P.vis, i0) && vis_bit(P.vis, i1) && vis_bit(P.vis, i2);
+    if (!SCATTER) {
+        stat_add(P, ST_FACES, 1);
+        if (!trivial) stat_add(P, ST_CLIPPED, 1);
+    }
+    if (trivial) {
+        TriS t;
+        if (tri_setup(P.vr[i0], P.vr[i1], P.vr[i2], P.width, P.height, t)) emit<SCATTER>(P, ws, t, (uint32_t)f);
+        return;
+    }
+    int pid;
+    if (!SCATTER) {
+        pid = clip_face(P, ws, f, i0, i1, i2);
+        P.poly_of_face[f] = pid;
+        if (pid >= 0) stat_add(P, ST_POLYS, 1);
+    } else {
+        pid = P.poly_of_face[f];
+    }
+    if (pid < 0) return;
+    int n = ws.pool_n[pid];
+    const float4* pv = ws.pool_vr + (size_t)pid * POLY_MAX;
+    for (int j = 0; j < n - 2; j++) {
+        TriS t;
+        if (tri_setup(pv[0], pv[j + 1], pv[j + 2], P.width, P.height, t))
+            emit<SCATTER>(P, ws, t, CLIPPED_BIT | ((uint32_t)pid << 3) | (uint32_t)j);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_scan: exclusive scan of the tile counts (one workgroup)
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(1024) void k_scan(Workspace ws) {
+    __shared__ uint32_t sh[1024];
+    int n = ws.ntiles, tid = threadIdx.x;
+    int per = (n + 1023) / 1024;
+    int b = tid * per, e = min(b + per, n);
+    uint32_t s = 0;
+    for (int i = b; i < e; i++) s += ws.tile_count[i];
+    sh[tid] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t v = tid >= off ? sh[tid - off] : 0u;
+        __syncthreads();
+        sh[tid] += v;
+        __syncthreads();
+    }
+    uint32_t run = sh[tid] - s;
+    for (int i = b; i < e; i++) {
+        ws.tile_offset[i] = run;
+        ws.tile_cursor[i] = run;
+        run += ws.tile_count[i];
+    }
+    if (tid == 1023) {
+        uint32_t total = sh[1023];
+        ws.tile_offset[n] = total;
+        ws.status[STATUS_BINS_NEEDED] = total;
+        if (total > ws.bin_cap) atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_BIN_OVERFLOW);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_raster: per-tile visibility in LDS, then one fragment program per pixel
+// ---------------------------------------------------------------------------
+
+constexpr int SMALL_PIX = 16;   // triangles with at most this many candidate pixels: one lane each
+
+struct TileLds {
+    uint64_t key[TILE_PIX];
+    uint32_t nan_order[TILE_PIX];   // 1 + order of the last NaN-depth fragment
+    float ax[BLOCK], ay[BLOCK], s0x[BLOCK], s0y[BLOCK], s1x[BLOCK], s1y[BLOCK], uz[BLOCK];
+    float z0[BLOCK], z1[BLOCK], z2[BLOCK];
+    uint32_t lo[BLOCK], box[BLOCK];
+    int big[BLOCK];
+    int nbig, head, nan_seen;
+};
+
+__device__ __forceinline__ bool load_tri(const PassDesc& P, const Workspace& ws, uint32_t e, TriS& t, uint32_t& order) {
+    float4 a, b, c;
+    if (!(e & CLIPPED_BIT)) {
+        int f = (int)e;
+        int i0 = P.idx[3 * f], i1 = P.idx[3 * f + 1], i2 = P.idx[3 * f + 2];
+        a = P.vr[i0]; b = P.vr[i1]; c = P.vr[i2];
+        order = (uint32_t)f << 4;
+    } else {
+        uint32_t pid = (e & ~CLIPPED_BIT) >> 3, j = e & 7u;
+        const float4* pv = ws.pool_vr + (size_t)pid * POLY_MAX;
+        a = pv[0]; b = pv[j + 1]; c = pv[j + 2];
+        order = ((uint32_t)ws.pool_face[pid] << 4) + 1u + j;
+    }
+    return tri_setup(a, b, c, P.width, P.height, t);
+}
+
+// One fragment of triangle t at tile-local pixel (lx, ly).
+__device__ __forceinline__ void frag_resolve(TileLds& L, const TriS& t, uint32_t lo, int xb, int yb, int lx, int ly,
+                                             bool second) {
+    float wx, wy, wz, z;
+    if (!tri_cover(t, xb + lx, yb + ly, wx, wy, wz, z)) return;
+    int i = (ly << TILE_SHIFT) | lx;
+    if (!second) {
+        if (z != z) {
+            atomicMax(&L.nan_order[i], (0xFFFFFFFEu - lo) + 1u);
+            L.nan_seen = 1;
+            return;
+        }
+        atomicMin((unsigned long long*)&L.key[i], ((uint64_t)depth_bits(z) << 32) | lo);
+    } else {
+        uint32_t no = L.nan_order[i];
+        if (no == 0u || z != z) return;
+        uint32_t order = 0xFFFFFFFEu - lo;
+        if (order + 1u <= no) return;   // only fragments after the last NaN one
+        atomicMin((unsigned long long*)&L.key[i], ((uint64_t)depth_bits(z) << 32) | lo);
+    }
+}
+
+// One sweep over the tile's bin: small triangles one per lane, larger ones
+// one per wave with the lanes spread over their candidate pixels.
+__device__ void sweep(TileLds& L, const PassDesc& P, const Workspace& ws, uint32_t beg, uint32_t cnt, int xb, int yb,
+                      bool second) {
+    int tid = threadIdx.x, lane = tid & 63;
+    for (uint32_t base = 0; base < cnt; base += BLOCK) {
+        bool ok = false;
+        TriS t;
+        uint32_t order = 0;
+        if (base + tid < cnt) ok = load_tri(P, ws, ws.bins[beg + base + tid], t, order);
+        int bx0 = 0, bx1 = -1, by0 = 0, by1 = -1;
+        if (ok) {
+            bx0 = max(t.x0, xb) - xb; bx1 = min(t.x1, xb + TILE - 1) - xb;
+            by0 = max(t.y0, yb) - yb; by1 = min(t.y1, yb + TILE - 1) - yb;
+            ok = bx1 >= bx0 && by1 >= by0;
+        }
+        uint32_t lo = 0xFFFFFFFEu - order;
+        if (ok) {
+            int npix = (bx1 - bx0 + 1) * (by1 - by0 + 1);
+            if (npix <= SMALL_PIX) {
+                for (int ly = by0; ly <= by1; ly++)
+                    for (int lx = bx0; lx <= bx1; lx++) frag_resolve(L, t, lo, xb, yb, lx, ly, second);
+            } else {
+                L.ax[tid] = t.ax; L.ay[tid] = t.ay; L.s0x[tid] = t.s0x; L.s0y[tid] = t.s0y;
+                L.s1x[tid] = t.s1x; L.s1y[tid] = t.s1y; L.uz[tid] = t.uz;
+                L.z0[tid] = t.z0; L.z1[tid] = t.z1; L.z2[tid] = t.z2;
+                L.lo[tid] = lo;
+                L.box[tid] = (uint32_t)bx0 | ((uint32_t)bx1 << 8) | ((uint32_t)by0 << 16) | ((uint32_t)by1 << 24);
+                int q = atomicAdd(&L.nbig, 1);
+                L.big[q] = tid;
+            }
+        }
+        __syncthreads();
+        for (;;) {
+            int q = 0;
+            if (lane == 0) q = atomicAdd(&L.head, 1);
+            q = __shfl(q, 0);
+            if (q >= L.nbig) break;
+            int s = L.big[q];
+            TriS bt;
+            bt.ax = L.ax[s]; bt.ay = L.ay[s]; bt.s0x = L.s0x[s]; bt.s0y = L.s0y[s];
+            bt.s1x = L.s1x[s]; bt.s1y = L.s1y[s]; bt.uz = L.uz[s];
+            bt.z0 = L.z0[s]; bt.z1 = L.z1[s]; bt.z2 = L.z2[s];
+            uint32_t blo = L.lo[s], box = L.box[s];
+            int x0 = box & 0xFF, x1 = (box >> 8) & 0xFF, y0 = (box >> 16) & 0xFF, y1 = box >> 24;
+            int wdt = x1 - x0 + 1;
+            int npix = wdt * (y1 - y0 + 1);
+            for (int k = lane; k < npix; k += 64) {
+                int ly = y0 + k / wdt, lx = x0 + k % wdt;
+                frag_resolve(L, bt, blo, xb, yb, lx, ly, second);
+            }
+        }
+        __syncthreads();
+        if (tid == 0) { L.nbig = 0; L.head = 0; }
+        __syncthreads();
+    }
+}
+
+// Interpolates, shades and writes the winning fragment of one pixel
+// (interpolate_varyings + fragment + set_depth/set_color, SR/graphics.cpp:205-220, 362-373).
+__device__ __forceinline__ void shade_pixel(const PassDesc& P, const Workspace& ws, uint32_t order, int px, int py) {
+    uint32_t f = order >> 4, slot = order & 15u;
+    float4 va, vb, vc;
+    const float *sa, *sb, *sc;
+    int nv;
+    if (slot == 0) {
+        int i0 = P.idx[3 * f], i1 = P.idx[3 * f + 1], i2 = P.idx[3 * f + 2];
+        va = P.vr[i0]; vb = P.vr[i1]; vc = P.vr[i2];
+        nv = P.nvary;
+        sa = P.vs + (size_t)i0 * nv; sb = P.vs + (size_t)i1 * nv; sc = P.vs + (size_t)i2 * nv;
+    } else {
+        int pid = P.poly_of_face[f];
+        uint32_t j = slot - 1;
+        size_t base = (size_t)pid * POLY_MAX;
+        va = ws.pool_vr[base]; vb = ws.pool_vr[base + j + 1]; vc = ws.pool_vr[base + j + 2];
+        nv = POLY_VARY;
+        sa = ws.pool_vs + base * POLY_VARY;
+        sb = ws.pool_vs + (base + j + 1) * POLY_VARY;
+        sc = ws.pool_vs + (base + j + 2) * POLY_VARY;
+    }
+    TriS t;
+    tri_setup(va, vb, vc, P.width, P.height, t);
+    float wx, wy, wz, z;
+    tri_cover(t, px, py, wx, wy, wz, z);
+    float k0 = va.w * wx, k1 = vb.w * wy, k2 = vc.w * wz;
+    float normalizer = 1 / (k0 + k1 + k2);
+    int cnt = varyings_of(P.kind);
+    float vy[8];
+    for (int c = 0; c < 8; c++) {
+        if (c < cnt) {
+            float sum = sa[c] * k0 + sb[c] * k1 + sc[c] * k2;
+            vy[c] = sum * normalizer;
+        } else {
+            vy[c] = 0.0f;
+        }
+    }
+    Tex dt{P.diffuse, P.dw, P.dh}, nt{P.normal, P.nw, P.nh};
+    ShadowMap sm{P.shadow, P.sw, P.sh};
+    C3 col = fragment_program(P.kind, *P.u, vy, dt, nt, sm);
+    size_t pix = (size_t)py * P.width + px;
+    uint32_t alpha = P.clear ? P.clear_rgba : P.color[pix];
+    P.color[pix] = pack_rgb(col, alpha);
+    P.depth[pix] = z;
+    if (P.winner) P.winner[pix] = order;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_raster(const PassDesc* passes, int npasses, Workspace ws) {
+    __shared__ TileLds L;
+    const PassDesc& P = passes[find_pass<2>(passes, npasses, blockIdx.x)];
+    int lt = blockIdx.x - P.tblock0;
+    int tx = lt % P.tiles_x, ty = lt / P.tiles_x;
+    int gt = P.tile0 + lt;
+    int xb = tx << TILE_SHIFT, yb = ty << TILE_SHIFT;
+    int tid = threadIdx.x;
+    int W = P.width, H = P.height;
+
+    for (int i = tid; i < TILE_PIX; i += BLOCK) {
+        int px = xb + (i & (TILE - 1)), py = yb + (i >> TILE_SHIFT);
+        uint64_t k = ~0ull;
+        if (px < W && py < H) k = init_key(P.clear ? P.clear_depth : P.depth[(size_t)py * W + px]);
+        L.key[i] = k;
+        L.nan_order[i] = 0u;
+    }
+    if (tid == 0) { L.nbig = 0; L.head = 0; L.nan_seen = 0; }
+    __syncthreads();
+
+    uint32_t beg = ws.tile_offset[gt];
+    uint32_t cnt = ws.tile_count[gt];
+    if (beg >= ws.bin_cap) cnt = 0;
+    else if (cnt > ws.bin_cap - beg) cnt = ws.bin_cap - beg;
+    sweep(L, P, ws, beg, cnt, xb, yb, false);
+
+    if (L.nan_seen) {
+        // pixels that saw a NaN-depth fragment restart after the last one
+        for (int i = tid; i < TILE_PIX; i += BLOCK) {
+            uint32_t no = L.nan_order[i];
+            if (no) L.key[i] = (0xFFFFFFFFull << 32) | (uint64_t)(0xFFFFFFFEu - (no - 1u));
+        }
+        __syncthreads();
+        sweep(L, P, ws, beg, cnt, xb, yb, true);
+    }
+
+    uint32_t written = 0;
+    for (int i = tid; i < TILE_PIX; i += BLOCK) {
+        int px = xb + (i & (TILE - 1)), py = yb + (i >> TILE_SHIFT);
+        if (px >= W || py >= H) continue;
+        uint32_t lo = (uint32_t)L.key[i];
+        size_t pix = (size_t)py * W + px;
+        if (lo == 0xFFFFFFFFu) {
+            if (P.clear) {
+                P.color[pix] = P.clear_rgba;
+                P.depth[pix] = P.clear_depth;
+            }
+            if (P.winner) P.winner[pix] = NO_WINNER;
+            continue;
+        }
+        shade_pixel(P, ws, 0xFFFFFFFEu - lo, px, py);
+        written++;
+    }
+    if (P.stats) stat_add(P, ST_PIXELS, written);
+}
+
+// fills a 32-bit buffer (graph-safe clear)
+__global__ void k_fill32(uint32_t* dst, uint32_t value, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = value;
+}
+
+// ---------------------------------------------------------------------------
+// launch wrappers
+// ---------------------------------------------------------------------------
+
+hipError_t launch_vertex(const PassDesc* d_passes, int npasses, const Workspace& ws, int blocks, hipStream_t s) {
+    if (blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_vertex, dim3(blocks), dim3(BLOCK), 0, s, d_passes, npasses, ws);
+    return hipGetLastError();
+}
+hipError_t launch_setup(const PassDesc* d_passes, int npasses, const Workspace& ws, int blocks, int scatter,
+                        hipStream_t s) {
+    if (blocks <= 0) return hipSuccess;
+    if (scatter) hipLaunchKernelGGL(k_setup<1>, dim3(blocks), dim3(BLOCK), 0, s, d_passes, npasses, ws);
+    else hipLaunchKernelGGL(k_setup<0>, dim3(blocks), dim3(BLOCK), 0, s, d_passes, npasses, ws);
+    return hipGetLastError();
+}
+hipError_t launch_scan(const Workspace& ws, hipStream_t s) {
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, ws);
+    return hipGetLastError();
+}
+hipError_t launch_raster(const PassDesc* d_passes, int npasses, const Workspace& ws, int blocks, hipStream_t s) {
+    if (blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_raster, dim3(blocks), dim3(BLOCK), 0, s, d_passes, npasses, ws);
+    return hipGetLastError();
+}
+hipError_t launch_fill32(uint32_t* dst, uint32_t value, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    int blocks = (int)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_fill32, dim3(blocks), dim3(256), 0, s, dst, value, n);
+    return hipGetLastError();
+}
+
+}  // namespace hana

@@ -1,0 +1,806 @@
+// hana_runtime.cpp -- the C ABI (include/hana_gpu.h): device objects, pass
+// planning, workspace management and the kernel launch sequence.
+//
+// A pass is one graphics_draw_triangle call (SR/graphics.cpp:378-407).  A
+// frame is DrawModel::draw (SR/scene.h:53-99): an optional shadow pass into
+// the shadow map, the main pass sampling it, then the shadow-map clear.  Any
+// set of passes is rendered by one launch sequence of five kernels over
+// per-pass descriptor tables (vertex, setup-count, scan, setup-scatter,
+// raster per dependency group), so a single frame and a batch of hundreds of
+// frames run the same code; batches are captured once into a hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hana_gpu.h"
+#include "hana_hostmath.h"
+#include "hana_launch.h"
+#include "hana_types.h"
+
+using namespace hana;
+
+struct hana_mesh {
+    int32_t V = 0, F = 0;
+    float* soa = nullptr;      // px | py | pz | nx | ny | nz | u | v, V floats each
+    int32_t* idx = nullptr;    // 3F
+};
+
+struct hana_texture {
+    int32_t w = 0, h = 0;
+    uint32_t* texels = nullptr;   // TGAColor bytes b,g,r,a per word
+};
+
+struct hana_target {
+    int32_t w = 0, h = 0;
+    uint32_t* color = nullptr;    // RGBA8 (R in the low byte)
+    float* depth = nullptr;
+    uint32_t* winner = nullptr;
+    bool pending_clear = false;
+    uint32_t clear_rgba = 0;
+    float clear_depth = 0;
+};
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct PassSpec {
+    hana_mesh* mesh;
+    int kind;
+    hana_uniforms u;
+    hana_texture* diffuse;
+    hana_texture* normal;
+    hana_target* shadow;    // sampled by the fragment program
+    hana_target* target;
+    int group;              // raster dependency group (shadow passes first)
+};
+
+// Grow-only device workspace for launch sequences.
+struct Arena {
+    DevBuf descs, unis, vr, vs, vis, pof, tile_count, tile_offset, tile_cursor, bins;
+    DevBuf pool_vr, pool_vs, pool_face, pool_n, status, stats;
+    uint32_t bin_cap = 0, poly_cap = 0;
+};
+
+struct Plan {
+    std::vector<PassSpec> specs;
+    std::vector<PassDesc> descs;
+    std::vector<DevUniforms> unis;
+    int vblocks = 0, fblocks = 0;
+    int ngroups = 0;
+    int group_first[2] = {0, 0}, group_count[2] = {0, 0}, group_blocks[2] = {0, 0};
+    size_t total_v = 0, total_vs = 0, total_vis = 0, total_f = 0;
+    int total_tiles = 0;
+    Workspace ws{};
+};
+
+constexpr int kRing = 32;
+
+}  // namespace
+
+struct hana_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool stats_enabled = false;
+    Arena arena;
+    // pinned staging ring for per-call descriptor uploads
+    void* ring_host[kRing] = {};
+    hipEvent_t ring_event[kRing] = {};
+    bool ring_used[kRing] = {};
+    size_t ring_bytes = 0;
+    int ring_next = 0;
+    // status of the last launch sequence
+    uint32_t* status_host = nullptr;   // pinned, STATUS_WORDS
+    uint64_t* stats_host = nullptr;    // pinned, per pass
+    int stats_passes = 0;
+    bool status_pending = false;
+};
+
+struct hana_batch {
+    std::vector<PassSpec> specs;
+    Plan plan;
+    Arena arena;
+    bool finalized = false;
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int fail(hana_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+int hip_fail(hana_ctx* c, hipError_t e, const char* what) {
+    return fail(c, HANA_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIP_TRY(c, expr)                                   \
+    do {                                                   \
+        hipError_t e_ = (expr);                            \
+        if (e_ != hipSuccess) return hip_fail(c, e_, #expr); \
+    } while (0)
+
+int ensure(hana_ctx* c, DevBuf& b, size_t bytes) {
+    if (bytes <= b.cap) return HANA_OK;
+    if (b.p) {
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        HIP_TRY(c, hipFree(b.p));
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    size_t nb = std::max(bytes, (size_t)256);
+    hipError_t e = hipMalloc(&b.p, nb);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(c, HANA_E_NOMEM, "device allocation of " + std::to_string(nb) + " bytes failed");
+    }
+    b.cap = nb;
+    return HANA_OK;
+}
+
+uint32_t pack_bytes(const uint8_t rgba[4]) {
+    return (uint32_t)rgba[0] | ((uint32_t)rgba[1] << 8) | ((uint32_t)rgba[2] << 16) | ((uint32_t)rgba[3] << 24);
+}
+
+int materialize_clear(hana_ctx* c, hana_target* t) {
+    if (!t->pending_clear) return HANA_OK;
+    size_t n = (size_t)t->w * t->h;
+    HIP_TRY(c, launch_fill32(t->color, t->clear_rgba, n, c->stream));
+    uint32_t db;
+    std::memcpy(&db, &t->clear_depth, 4);
+    HIP_TRY(c, launch_fill32(reinterpret_cast<uint32_t*>(t->depth), db, n, c->stream));
+    t->pending_clear = false;
+    return HANA_OK;
+}
+
+// Lays out the workspace of a set of passes.  Passes are ordered by raster
+// group (stable); every per-pass slice is 256-vertex aligned.
+int build_plan(hana_ctx* c, std::vector<PassSpec> specs, Plan& pl) {
+    std::stable_sort(specs.begin(), specs.end(), [](const PassSpec& a, const PassSpec& b) { return a.group < b.group; });
+    pl = Plan();
+    pl.specs = specs;
+    int n = (int)specs.size();
+    pl.descs.resize(n);
+    pl.unis.resize(n);
+    size_t voff = 0, vsoff = 0, fo = 0;
+    int vb = 0, fb = 0, tile0 = 0;
+    int tb[2] = {0, 0};
+    for (int i = 0; i < n; i++) {
+        const PassSpec& s = specs[i];
+        if (s.group < 0 || s.group > 1) return fail(c, HANA_E_INVALID, "bad pass group");
+        PassDesc& d = pl.descs[i];
+        std::memset(&d, 0, sizeof d);
+        derive_dev_uniforms(s.u, pl.unis[i]);
+        hana_mesh* m = s.mesh;
+        d.nverts = m->V;
+        d.nfaces = m->F;
+        d.kind = s.kind;
+        d.nvary = (s.kind == SHADER_GROUND || s.kind == SHADER_TOON || s.kind == SHADER_SHADOW) ? 1
+                  : (s.kind == SHADER_TEXTURE ? 2 : 8);
+        size_t V = (size_t)m->V;
+        d.px = m->soa; d.py = m->soa + V; d.pz = m->soa + 2 * V;
+        d.nx = m->soa + 3 * V; d.ny = m->soa + 4 * V; d.nz = m->soa + 5 * V;
+        d.tu = m->soa + 6 * V; d.tv = m->soa + 7 * V;
+        d.idx = m->idx;
+        if (s.diffuse) { d.diffuse = s.diffuse->texels; d.dw = s.diffuse->w; d.dh = s.diffuse->h; }
+        if (s.normal) { d.normal = s.normal->texels; d.nw = s.normal->w; d.nh = s.normal->h; }
+        if (s.shadow) { d.shadow = s.shadow->color; d.sw = s.shadow->w; d.sh = s.shadow->h; }
+        hana_target* t = s.target;
+        d.color = t->color; d.depth = t->depth; d.winner = t->winner;
+        d.width = t->w; d.height = t->h;
+        d.tiles_x = (t->w + TILE - 1) / TILE;
+        d.tiles_y = (t->h + TILE - 1) / TILE;
+        size_t vpad = (V + BLOCK - 1) / BLOCK * BLOCK;
+        // slices (pointers filled after allocation): store offsets in the pointer fields
+        d.vr = reinterpret_cast<float4*>(voff);
+        d.vs = reinterpret_cast<float*>(vsoff);
+        d.vis = reinterpret_cast<uint64_t*>(voff / 64);
+        d.poly_of_face = reinterpret_cast<int32_t*>(fo);
+        voff += vpad;
+        vsoff += vpad * (size_t)d.nvary;
+        fo += (size_t)m->F;
+        d.vblock0 = vb;
+        vb += (int)(vpad / BLOCK);
+        d.fblock0 = fb;
+        fb += (m->F + BLOCK - 1) / BLOCK;
+        d.tile0 = tile0;
+        tile0 += d.tiles_x * d.tiles_y;
+        d.tblock0 = tb[s.group];
+        tb[s.group] += d.tiles_x * d.tiles_y;
+    }
+    pl.vblocks = vb;
+    pl.fblocks = fb;
+    pl.total_v = voff;
+    pl.total_vs = vsoff;
+    pl.total_vis = voff / 64;
+    pl.total_f = fo;
+    pl.total_tiles = tile0;
+    pl.ngroups = 0;
+    for (int g = 0; g < 2; g++) {
+        int first = -1, count = 0;
+        for (int i = 0; i < n; i++)
+            if (specs[i].group == g) {
+                if (first < 0) first = i;
+                count++;
+            }
+        pl.group_first[g] = first < 0 ? 0 : first;
+        pl.group_count[g] = count;
+        pl.group_blocks[g] = tb[g];
+    }
+    return HANA_OK;
+}
+
+// Allocates the arena for a plan and resolves the slice pointers.
+int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_polys, bool with_stats) {
+    int n = (int)pl.descs.size();
+    int r;
+    if ((r = ensure(c, A.descs, sizeof(PassDesc) * std::max(n, 1)))) return r;
+    if ((r = ensure(c, A.unis, sizeof(DevUniforms) * std::max(n, 1)))) return r;
+    if ((r = ensure(c, A.vr, sizeof(float4) * std::max<size_t>(pl.total_v, 1)))) return r;
+    if ((r = ensure(c, A.vs, sizeof(float) * std::max<size_t>(pl.total_vs, 1)))) return r;
+    if ((r = ensure(c, A.vis, sizeof(uint64_t) * std::max<size_t>(pl.total_vis, 1)))) return r;
+    if ((r = ensure(c, A.pof, sizeof(int32_t) * std::max<size_t>(pl.total_f, 1)))) return r;
+    size_t nt = (size_t)pl.total_tiles + 1;
+    if ((r = ensure(c, A.tile_count, sizeof(uint32_t) * nt))) return r;
+    if ((r = ensure(c, A.tile_offset, sizeof(uint32_t) * nt))) return r;
+    if ((r = ensure(c, A.tile_cursor, sizeof(uint32_t) * nt))) return r;
+    // (triangle, tile) entries: most triangles touch one tile and about half
+    // are culled; the scan reports the exact need if this is exceeded
+    uint64_t want_bins = std::max<uint64_t>({(uint64_t)min_bins, (uint64_t)4 * pl.total_f + 2 * nt, 1u << 22});
+    want_bins = std::min<uint64_t>(want_bins, 0xFFFFFFF0ull);
+    if (want_bins > A.bin_cap) {
+        if ((r = ensure(c, A.bins, sizeof(uint32_t) * want_bins))) return r;
+        A.bin_cap = (uint32_t)want_bins;
+    }
+    uint64_t want_polys = std::max<uint64_t>({(uint64_t)min_polys, pl.total_f / 16 + 4096});
+    want_polys = std::min<uint64_t>(want_polys, (1u << 28) - 1);
+    if (want_polys > A.poly_cap) {
+        if ((r = ensure(c, A.pool_vr, sizeof(float4) * POLY_MAX * want_polys))) return r;
+        if ((r = ensure(c, A.pool_vs, sizeof(float) * POLY_MAX * POLY_VARY * want_polys))) return r;
+        if ((r = ensure(c, A.pool_face, sizeof(int32_t) * want_polys))) return r;
+        if ((r = ensure(c, A.pool_n, sizeof(int32_t) * want_polys))) return r;
+        A.poly_cap = (uint32_t)want_polys;
+    }
+    if ((r = ensure(c, A.status, sizeof(uint32_t) * STATUS_WORDS))) return r;
+    if ((r = ensure(c, A.stats, sizeof(uint64_t) * ST_WORDS * std::max(n, 1)))) return r;
+    for (int i = 0; i < n; i++) {
+        PassDesc& d = pl.descs[i];
+        size_t voff = reinterpret_cast<size_t>(d.vr);
+        size_t vsoff = reinterpret_cast<size_t>(d.vs);
+        size_t fo = reinterpret_cast<size_t>(d.poly_of_face);
+        d.vr = A.vr.as<float4>() + voff;
+        d.vs = A.vs.as<float>() + vsoff;
+        d.vis = A.vis.as<uint64_t>() + voff / 64;
+        d.poly_of_face = A.pof.as<int32_t>() + fo;
+        d.u = A.unis.as<DevUniforms>() + i;
+        d.stats = with_stats ? A.stats.as<uint64_t>() + (size_t)ST_WORDS * i : nullptr;
+    }
+    Workspace& ws = pl.ws;
+    ws.tile_count = A.tile_count.as<uint32_t>();
+    ws.tile_offset = A.tile_offset.as<uint32_t>();
+    ws.tile_cursor = A.tile_cursor.as<uint32_t>();
+    ws.bins = A.bins.as<uint32_t>();
+    ws.pool_vr = A.pool_vr.as<float4>();
+    ws.pool_vs = A.pool_vs.as<float>();
+    ws.pool_face = A.pool_face.as<int32_t>();
+    ws.pool_n = A.pool_n.as<int32_t>();
+    ws.status = A.status.as<uint32_t>();
+    ws.ntiles = pl.total_tiles;
+    ws.bin_cap = A.bin_cap;
+    ws.poly_cap = A.poly_cap;
+    return HANA_OK;
+}
+
+// Resolves clear flags of the plan's targets (fused into their first pass).
+void take_clears(Plan& pl) {
+    for (size_t i = 0; i < pl.descs.size(); i++) {
+        hana_target* t = pl.specs[i].target;
+        PassDesc& d = pl.descs[i];
+        d.clear = 0;
+        if (t->pending_clear) {
+            d.clear = 1;
+            d.clear_rgba = t->clear_rgba;
+            d.clear_depth = t->clear_depth;
+            t->pending_clear = false;
+        }
+    }
+}
+
+int enqueue_sequence(hana_ctx* c, const Plan& pl, const Arena& A, hipStream_t s) {
+    const PassDesc* d = A.descs.as<PassDesc>();
+    int n = (int)pl.descs.size();
+    HIP_TRY(c, launch_vertex(d, n, pl.ws, pl.vblocks, s));
+    HIP_TRY(c, launch_setup(d, n, pl.ws, pl.fblocks, 0, s));
+    HIP_TRY(c, launch_scan(pl.ws, s));
+    HIP_TRY(c, launch_setup(d, n, pl.ws, pl.fblocks, 1, s));
+    for (int g = 0; g < 2; g++)
+        if (pl.group_count[g])
+            HIP_TRY(c, launch_raster(d + pl.group_first[g], pl.group_count[g], pl.ws, pl.group_blocks[g], s));
+    return HANA_OK;
+}
+
+// Uploads descriptors and uniforms through a pinned staging slot.
+int upload_tables(hana_ctx* c, const Plan& pl, Arena& A) {
+    size_t nd = sizeof(PassDesc) * pl.descs.size(), nu = sizeof(DevUniforms) * pl.unis.size();
+    size_t need = nd + nu;
+    if (need > c->ring_bytes) {
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        for (int i = 0; i < kRing; i++) {
+            if (c->ring_host[i]) HIP_TRY(c, hipHostFree(c->ring_host[i]));
+            HIP_TRY(c, hipHostMalloc(&c->ring_host[i], need * 2));
+            c->ring_used[i] = false;
+        }
+        c->ring_bytes = need * 2;
+    }
+    int slot = c->ring_next;
+    c->ring_next = (c->ring_next + 1) % kRing;
+    if (c->ring_used[slot]) HIP_TRY(c, hipEventSynchronize(c->ring_event[slot]));
+    char* h = static_cast<char*>(c->ring_host[slot]);
+    std::memcpy(h, pl.descs.data(), nd);
+    std::memcpy(h + nd, pl.unis.data(), nu);
+    HIP_TRY(c, hipMemcpyAsync(A.descs.p, h, nd, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(A.unis.p, h + nd, nu, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipEventRecord(c->ring_event[slot], c->stream));
+    c->ring_used[slot] = true;
+    return HANA_OK;
+}
+
+int check_status(hana_ctx* c) {
+    if (!c->status_pending) return HANA_OK;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->status_pending = false;
+    uint32_t flags = c->status_host[STATUS_FLAGS];
+    if (flags & FLAG_POLY_TOO_LARGE)
+        return fail(c, HANA_E_OVERFLOW, "a clipped polygon exceeded 10 vertices and was dropped");
+    if (flags) {
+        uint32_t need_bins = c->status_host[STATUS_BINS_NEEDED];
+        uint32_t need_polys = c->status_host[STATUS_POLYS];
+        // grow for the next call; the frame that overflowed must be re-rendered
+        if (flags & FLAG_BIN_OVERFLOW) {
+            uint64_t want = (uint64_t)need_bins + need_bins / 4 + 1024;
+            int r = ensure(c, c->arena.bins, sizeof(uint32_t) * want);
+            if (r) return r;
+            c->arena.bin_cap = (uint32_t)std::min<uint64_t>(want, 0xFFFFFFF0ull);
+        }
+        if (flags & FLAG_POLY_OVERFLOW) {
+            uint64_t want = (uint64_t)need_polys + need_polys / 4 + 1024;
+            int r;
+            if ((r = ensure(c, c->arena.pool_vr, sizeof(float4) * POLY_MAX * want))) return r;
+            if ((r = ensure(c, c->arena.pool_vs, sizeof(float) * POLY_MAX * POLY_VARY * want))) return r;
+            if ((r = ensure(c, c->arena.pool_face, sizeof(int32_t) * want))) return r;
+            if ((r = ensure(c, c->arena.pool_n, sizeof(int32_t) * want))) return r;
+            c->arena.poly_cap = (uint32_t)want;
+        }
+        return fail(c, HANA_E_OVERFLOW,
+                    "bin or clip-pool capacity exceeded by the last launch; buffers were grown, re-render the frame");
+    }
+    return HANA_OK;
+}
+
+// Renders a list of passes with the context's arena (per-call path).
+int run_passes(hana_ctx* c, std::vector<PassSpec> specs) {
+    int r = check_status(c);
+    if (r) return r;
+    for (auto& s : specs) {
+        if (!s.shadow || !s.shadow->pending_clear) continue;
+        bool drawn = false;
+        for (auto& o : specs) drawn = drawn || o.target == s.shadow;
+        if (!drawn && (r = materialize_clear(c, s.shadow))) return r;   // sampled, not drawn here
+    }
+    Plan pl;
+    if ((r = build_plan(c, std::move(specs), pl))) return r;
+    if ((r = bind_plan(c, pl, c->arena, 0, 0, c->stats_enabled))) return r;
+    // a target sampled as a shadow map by a pass of the same plan must be drawn
+    // by an earlier group; clears fuse into the first pass writing the target
+    take_clears(pl);
+    if ((r = upload_tables(c, pl, c->arena))) return r;
+    int n = (int)pl.descs.size();
+    if (c->stats_enabled) {
+        HIP_TRY(c, launch_fill32(reinterpret_cast<uint32_t*>(c->arena.stats.p), 0u, (size_t)ST_WORDS * 2 * n,
+                                 c->stream));
+    }
+    if ((r = enqueue_sequence(c, pl, c->arena, c->stream))) return r;
+    HIP_TRY(c, hipMemcpyAsync(c->status_host, c->arena.status.p, sizeof(uint32_t) * STATUS_WORDS,
+                              hipMemcpyDeviceToHost, c->stream));
+    if (c->stats_enabled) {
+        if (c->stats_passes < n) {
+            if (c->stats_host) HIP_TRY(c, hipHostFree(c->stats_host));
+            HIP_TRY(c, hipHostMalloc((void**)&c->stats_host, sizeof(uint64_t) * ST_WORDS * n));
+            c->stats_passes = n;
+        }
+        HIP_TRY(c, hipMemcpyAsync(c->stats_host, c->arena.stats.p, sizeof(uint64_t) * ST_WORDS * n,
+                                  hipMemcpyDeviceToHost, c->stream));
+        c->stats_passes = n;
+    }
+    c->status_pending = true;
+    return HANA_OK;
+}
+
+bool valid_kind(int k) { return k >= HANA_SHADER_GROUND && k <= HANA_SHADER_SHADOW; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int hana_gpu_create(int device, hana_ctx** out) {
+    if (!out) return HANA_E_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return HANA_E_DEVICE;
+    if (device < 0 || device >= count) return HANA_E_INVALID;
+    hana_ctx* c = new hana_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return HANA_E_DEVICE;
+    }
+    for (int i = 0; i < kRing; i++) {
+        if (hipEventCreateWithFlags(&c->ring_event[i], hipEventDisableTiming) != hipSuccess) {
+            delete c;
+            return HANA_E_DEVICE;
+        }
+    }
+    if (hipHostMalloc((void**)&c->status_host, sizeof(uint32_t) * STATUS_WORDS) != hipSuccess) {
+        delete c;
+        return HANA_E_DEVICE;
+    }
+    std::memset(c->status_host, 0, sizeof(uint32_t) * STATUS_WORDS);
+    *out = c;
+    return HANA_OK;
+}
+
+static void free_arena(Arena& a) {
+    DevBuf* bufs[] = {&a.descs, &a.unis, &a.vr, &a.vs, &a.vis, &a.pof, &a.tile_count, &a.tile_offset,
+                      &a.tile_cursor, &a.bins, &a.pool_vr, &a.pool_vs, &a.pool_face, &a.pool_n, &a.status, &a.stats};
+    for (DevBuf* b : bufs)
+        if (b->p) {
+            (void)hipFree(b->p);
+            b->p = nullptr;
+            b->cap = 0;
+        }
+}
+
+int hana_gpu_destroy(hana_ctx* c) {
+    if (!c) return HANA_E_INVALID;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    free_arena(c->arena);
+    for (int i = 0; i < kRing; i++) {
+        if (c->ring_host[i]) (void)hipHostFree(c->ring_host[i]);
+        if (c->ring_event[i]) (void)hipEventDestroy(c->ring_event[i]);
+    }
+    if (c->status_host) (void)hipHostFree(c->status_host);
+    if (c->stats_host) (void)hipHostFree(c->stats_host);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return HANA_OK;
+}
+
+const char* hana_gpu_last_error(hana_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+int hana_gpu_sync(hana_ctx* c) {
+    if (!c) return HANA_E_INVALID;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return check_status(c);
+}
+
+int hana_gpu_set_stats(hana_ctx* c, int enable) {
+    if (!c) return HANA_E_INVALID;
+    c->stats_enabled = enable != 0;
+    return HANA_OK;
+}
+
+int hana_gpu_last_stats(hana_ctx* c, hana_pass_stats* out, int max_passes, int* n_passes) {
+    if (!c || !out || !n_passes) return HANA_E_INVALID;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    int n = std::min(max_passes, c->stats_host ? c->stats_passes : 0);
+    for (int i = 0; i < n; i++) {
+        const uint64_t* s = c->stats_host + (size_t)ST_WORDS * i;
+        out[i].faces = (int64_t)s[ST_FACES];
+        out[i].clipped_faces = (int64_t)s[ST_CLIPPED];
+        out[i].polygons = (int64_t)s[ST_POLYS];
+        out[i].tile_entries = (int64_t)s[ST_ENTRIES];
+        out[i].pixels_written = (int64_t)s[ST_PIXELS];
+    }
+    *n_passes = n;
+    return HANA_OK;
+}
+
+int hana_mesh_create(hana_ctx* c, int32_t nverts, const float* pos, const float* nrm, const float* uv,
+                     int32_t nfaces, const int32_t* idx, hana_mesh** out) {
+    if (!c || !out || nverts < 0 || nfaces < 0 || (nverts && !pos) || (nfaces && !idx))
+        return fail(c, HANA_E_INVALID, "hana_mesh_create: bad arguments");
+    if (nfaces > (1 << 27)) return fail(c, HANA_E_INVALID, "hana_mesh_create: at most 2^27 faces");
+    for (int64_t i = 0; i < 3 * (int64_t)nfaces; i++)
+        if (idx[i] < 0 || idx[i] >= nverts) return fail(c, HANA_E_INVALID, "hana_mesh_create: index out of range");
+    size_t V = (size_t)nverts;
+    std::vector<float> soa(8 * std::max<size_t>(V, 1), 0.0f);
+    for (size_t i = 0; i < V; i++) {
+        for (int k = 0; k < 3; k++) soa[k * V + i] = pos[3 * i + k];
+        if (nrm)
+            for (int k = 0; k < 3; k++) soa[(3 + k) * V + i] = nrm[3 * i + k];
+        if (uv)
+            for (int k = 0; k < 2; k++) soa[(6 + k) * V + i] = uv[2 * i + k];
+    }
+    hana_mesh* m = new hana_mesh();
+    m->V = nverts;
+    m->F = nfaces;
+    if (hipMalloc((void**)&m->soa, sizeof(float) * soa.size()) != hipSuccess ||
+        hipMalloc((void**)&m->idx, sizeof(int32_t) * 3 * std::max<size_t>(nfaces, 1)) != hipSuccess) {
+        (void)hipGetLastError();
+        if (m->soa) (void)hipFree(m->soa);
+        delete m;
+        return fail(c, HANA_E_NOMEM, "hana_mesh_create: device allocation failed");
+    }
+    HIP_TRY(c, hipMemcpy(m->soa, soa.data(), sizeof(float) * soa.size(), hipMemcpyHostToDevice));
+    if (nfaces) HIP_TRY(c, hipMemcpy(m->idx, idx, sizeof(int32_t) * 3 * (size_t)nfaces, hipMemcpyHostToDevice));
+    *out = m;
+    return HANA_OK;
+}
+
+int hana_mesh_destroy(hana_ctx* c, hana_mesh* m) {
+    if (!c || !m) return HANA_E_INVALID;
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(m->soa);
+    (void)hipFree(m->idx);
+    delete m;
+    return HANA_OK;
+}
+
+int hana_texture_create(hana_ctx* c, int32_t w, int32_t h, int32_t bpp, const uint8_t* bytes, hana_texture** out) {
+    if (!c || !out || w <= 0 || h <= 0 || !(bpp == 1 || bpp == 3 || bpp == 4) || !bytes)
+        return fail(c, HANA_E_INVALID, "hana_texture_create: bad arguments");
+    size_t n = (size_t)w * h;
+    std::vector<uint32_t> words(n);
+    for (size_t i = 0; i < n; i++) {
+        uint32_t v = 0;
+        for (int k = 0; k < bpp; k++) v |= (uint32_t)bytes[i * bpp + k] << (8 * k);   // TGAColor(p, bpp)
+        words[i] = v;
+    }
+    hana_texture* t = new hana_texture();
+    t->w = w;
+    t->h = h;
+    if (hipMalloc((void**)&t->texels, sizeof(uint32_t) * n) != hipSuccess) {
+        (void)hipGetLastError();
+        delete t;
+        return fail(c, HANA_E_NOMEM, "hana_texture_create: device allocation failed");
+    }
+    HIP_TRY(c, hipMemcpy(t->texels, words.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    *out = t;
+    return HANA_OK;
+}
+
+int hana_texture_destroy(hana_ctx* c, hana_texture* t) {
+    if (!c || !t) return HANA_E_INVALID;
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(t->texels);
+    delete t;
+    return HANA_OK;
+}
+
+int hana_target_create(hana_ctx* c, int32_t w, int32_t h, hana_target** out) {
+    if (!c || !out || w <= 0 || h <= 0 || (int64_t)w * h > (1ll << 30))
+        return fail(c, HANA_E_INVALID, "hana_target_create: bad size");
+    hana_target* t = new hana_target();
+    t->w = w;
+    t->h = h;
+    size_t n = (size_t)w * h;
+    if (hipMalloc((void**)&t->color, 4 * n) != hipSuccess || hipMalloc((void**)&t->depth, 4 * n) != hipSuccess) {
+        (void)hipGetLastError();
+        if (t->color) (void)hipFree(t->color);
+        delete t;
+        return fail(c, HANA_E_NOMEM, "hana_target_create: device allocation failed");
+    }
+    // RenderBuffer constructor state: colour {0,0,0,1} -> (0,0,0,255), depth 1
+    // (SR/renderbuffer.cpp:3-18)
+    t->pending_clear = true;
+    t->clear_rgba = 0xFF000000u;
+    t->clear_depth = 1.0f;
+    *out = t;
+    return HANA_OK;
+}
+
+int hana_target_destroy(hana_ctx* c, hana_target* t) {
+    if (!c || !t) return HANA_E_INVALID;
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(t->color);
+    (void)hipFree(t->depth);
+    if (t->winner) (void)hipFree(t->winner);
+    delete t;
+    return HANA_OK;
+}
+
+int hana_target_clear(hana_ctx* c, hana_target* t, const uint8_t rgba[4], float depth) {
+    if (!c || !t || !rgba) return HANA_E_INVALID;
+    t->pending_clear = true;
+    t->clear_rgba = pack_bytes(rgba);
+    t->clear_depth = depth;
+    return HANA_OK;
+}
+
+int hana_target_upload(hana_ctx* c, hana_target* t, const uint8_t* rgba, const float* depth) {
+    if (!c || !t) return HANA_E_INVALID;
+    size_t n = (size_t)t->w * t->h;
+    int r = materialize_clear(c, t);
+    if (r) return r;
+    if (rgba) HIP_TRY(c, hipMemcpyAsync(t->color, rgba, 4 * n, hipMemcpyHostToDevice, c->stream));
+    if (depth) HIP_TRY(c, hipMemcpyAsync(t->depth, depth, 4 * n, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return HANA_OK;
+}
+
+int hana_target_read(hana_ctx* c, hana_target* t, uint8_t* rgba, float* depth) {
+    if (!c || !t) return HANA_E_INVALID;
+    int r = materialize_clear(c, t);
+    if (r) return r;
+    size_t n = (size_t)t->w * t->h;
+    if (rgba) HIP_TRY(c, hipMemcpyAsync(rgba, t->color, 4 * n, hipMemcpyDeviceToHost, c->stream));
+    if (depth) HIP_TRY(c, hipMemcpyAsync(depth, t->depth, 4 * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return check_status(c);
+}
+
+int hana_target_enable_winner(hana_ctx* c, hana_target* t, int enable) {
+    if (!c || !t) return HANA_E_INVALID;
+    if (enable && !t->winner) {
+        size_t n = (size_t)t->w * t->h;
+        if (hipMalloc((void**)&t->winner, 4 * n) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(c, HANA_E_NOMEM, "winner map allocation failed");
+        }
+        HIP_TRY(c, launch_fill32(t->winner, NO_WINNER, n, c->stream));
+    } else if (!enable && t->winner) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(t->winner);
+        t->winner = nullptr;
+    }
+    return HANA_OK;
+}
+
+int hana_target_read_winner(hana_ctx* c, hana_target* t, uint32_t* winner) {
+    if (!c || !t || !winner || !t->winner) return HANA_E_INVALID;
+    size_t n = (size_t)t->w * t->h;
+    HIP_TRY(c, hipMemcpyAsync(winner, t->winner, 4 * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return HANA_OK;
+}
+
+int hana_draw(hana_ctx* c, const hana_draw_desc* d) {
+    if (!c || !d || !d->mesh || !d->target || !d->uniforms || !valid_kind(d->shader))
+        return fail(c, HANA_E_INVALID, "hana_draw: bad descriptor");
+    if (d->shadow_map == d->target) return fail(c, HANA_E_INVALID, "hana_draw: shadow map is the target");
+    PassSpec s{d->mesh, d->shader, *d->uniforms, d->diffuse, d->normal, d->shadow_map, d->target, 0};
+    return run_passes(c, {s});
+}
+
+int hana_render_frame(hana_ctx* c, const hana_frame_desc* f) {
+    if (!c || !f || !f->mesh || !f->target || !f->uniforms || !valid_kind(f->shader) ||
+        (f->enable_shadow && (!f->shadow_map || f->shadow_map == f->target)))
+        return fail(c, HANA_E_INVALID, "hana_render_frame: bad descriptor");
+    hana_uniforms u = *f->uniforms;
+    u.enable_shadow = f->enable_shadow;
+    std::vector<PassSpec> specs;
+    if (f->enable_shadow) specs.push_back({f->mesh, HANA_SHADER_SHADOW, u, f->diffuse, f->normal, nullptr, f->shadow_map, 0});
+    specs.push_back({f->mesh, f->shader, u, f->diffuse, f->normal, f->enable_shadow ? f->shadow_map : nullptr, f->target,
+                     f->enable_shadow ? 1 : 0});
+    int r = run_passes(c, std::move(specs));
+    if (r) return r;
+    if (f->enable_shadow) {
+        // shdaow_map->renderbuffer_clear_color(Black) / clear_depth(FLT_MAX)
+        // after the main pass (SR/scene.h:94-98); alpha is written as 255
+        f->shadow_map->pending_clear = true;
+        f->shadow_map->clear_rgba = 0xFF000000u;
+        f->shadow_map->clear_depth = 3.40282347e+38f;
+    }
+    return HANA_OK;
+}
+
+// ---- batches ----
+
+int hana_batch_create(hana_ctx* c, hana_batch** out) {
+    if (!c || !out) return HANA_E_INVALID;
+    *out = new hana_batch();
+    return HANA_OK;
+}
+
+int hana_batch_add_frame(hana_ctx* c, hana_batch* b, const hana_frame_desc* f) {
+    if (!c || !b || !f || !f->mesh || !f->target || !f->uniforms || !valid_kind(f->shader) ||
+        (f->enable_shadow && (!f->shadow_map || f->shadow_map == f->target)))
+        return fail(c, HANA_E_INVALID, "hana_batch_add_frame: bad descriptor");
+    if (b->finalized) return fail(c, HANA_E_INVALID, "hana_batch_add_frame: batch already finalized");
+    hana_uniforms u = *f->uniforms;
+    u.enable_shadow = f->enable_shadow;
+    if (f->enable_shadow) b->specs.push_back({f->mesh, HANA_SHADER_SHADOW, u, f->diffuse, f->normal, nullptr, f->shadow_map, 0});
+    b->specs.push_back({f->mesh, f->shader, u, f->diffuse, f->normal, f->enable_shadow ? f->shadow_map : nullptr,
+                        f->target, 1});
+    return HANA_OK;
+}
+
+// Every frame of a batch clears its target and shadow map first (the state
+// DrawModel::draw runs in after main.cpp's per-frame clear).
+static void batch_clear_flags(hana_batch* b) {
+    for (size_t i = 0; i < b->plan.descs.size(); i++) {
+        PassDesc& d = b->plan.descs[i];
+        d.clear = 1;
+        d.clear_rgba = 0xFF000000u;
+        d.clear_depth = 3.40282347e+38f;
+    }
+}
+
+int hana_batch_finalize(hana_ctx* c, hana_batch* b, int use_graph) {
+    if (!c || !b) return HANA_E_INVALID;
+    if (b->specs.empty()) return fail(c, HANA_E_INVALID, "hana_batch_finalize: empty batch");
+    for (size_t i = 0; i < b->specs.size(); i++)
+        for (size_t j = i + 1; j < b->specs.size(); j++)
+            if (b->specs[i].target == b->specs[j].target)
+                return fail(c, HANA_E_INVALID, "hana_batch_finalize: frames of a batch need distinct targets and shadow maps");
+    int r;
+    if ((r = check_status(c))) return r;
+    if ((r = build_plan(c, b->specs, b->plan))) return r;
+    uint32_t min_bins = 0, min_polys = 0;
+    for (int attempt = 0; attempt < 3; attempt++) {
+        if ((r = bind_plan(c, b->plan, b->arena, min_bins, min_polys, false))) return r;
+        batch_clear_flags(b);
+        HIP_TRY(c, hipMemcpy(b->arena.descs.p, b->plan.descs.data(), sizeof(PassDesc) * b->plan.descs.size(),
+                             hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(b->arena.unis.p, b->plan.unis.data(), sizeof(DevUniforms) * b->plan.unis.size(),
+                             hipMemcpyHostToDevice));
+        // trial run: sizes the bins and the clip pool exactly
+        if ((r = enqueue_sequence(c, b->plan, b->arena, c->stream))) return r;
+        uint32_t st[STATUS_WORDS];
+        HIP_TRY(c, hipMemcpyAsync(st, b->arena.status.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        if (!st[STATUS_FLAGS]) break;
+        min_bins = st[STATUS_BINS_NEEDED] + st[STATUS_BINS_NEEDED] / 8 + 1024;
+        min_polys = st[STATUS_POLYS] + st[STATUS_POLYS] / 8 + 1024;
+        if (attempt == 2) return fail(c, HANA_E_OVERFLOW, "hana_batch_finalize: capacity not reached");
+    }
+    b->use_graph = use_graph != 0;
+    if (b->use_graph) {
+        HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int er = enqueue_sequence(c, b->plan, b->arena, c->stream);
+        hipGraph_t g = nullptr;
+        hipError_t ec = hipStreamEndCapture(c->stream, &g);
+        if (er) return er;
+        if (ec != hipSuccess) return hip_fail(c, ec, "hipStreamEndCapture");
+        b->graph = g;
+        HIP_TRY(c, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    b->finalized = true;
+    return HANA_OK;
+}
+
+int hana_batch_run(hana_ctx* c, hana_batch* b) {
+    if (!c || !b || !b->finalized) return fail(c, HANA_E_INVALID, "hana_batch_run: batch not finalized");
+    if (b->use_graph) {
+        HIP_TRY(c, hipGraphLaunch(b->exec, c->stream));
+        return HANA_OK;
+    }
+    return enqueue_sequence(c, b->plan, b->arena, c->stream);
+}
+
+int hana_batch_destroy(hana_ctx* c, hana_batch* b) {
+    if (!c || !b) return HANA_E_INVALID;
+    (void)hipStreamSynchronize(c->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    free_arena(b->arena);
+    delete b;
+    return HANA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,100 @@
+// hana_types.h -- data layouts shared by the host runtime and the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hana {
+
+enum {
+    SHADER_GROUND = 0,
+    SHADER_TOON = 1,
+    SHADER_TEXTURE = 2,
+    SHADER_TEXTURE_LIGHT = 3,
+    SHADER_BLINN = 4,
+    SHADER_NORMALMAP = 5,
+    SHADER_SHADOW = 6
+};
+
+constexpr int TILE = 32;                   // screen tile edge (pixels)
+constexpr int TILE_SHIFT = 5;
+constexpr int TILE_PIX = TILE * TILE;
+constexpr int BLOCK = 256;                 // threads per workgroup (4 wave64)
+constexpr int POLY_MAX = 10;               // vertices of a clipped polygon (3 + 7 planes)
+constexpr int POLY_VARY = 8;               // varyings stored per clipped vertex
+constexpr uint32_t NO_WINNER = 0xFFFFFFFFu;
+constexpr uint32_t CLIPPED_BIT = 0x80000000u;
+
+// Uniform block of one pass, derived on the host from ShaderData
+// (SR/IShader.h:17-32) with the products the reference recomputes per vertex
+// folded once: clip_m = camera_vp*model (ObjectToClipPos, SR/IShader.h:55-57),
+// light_m = light_vp*model (ObjectToViewPos, SR/IShader.h:59-61).
+struct DevUniforms {
+    float clip_m[16];
+    float light_m[16];
+    float model[16];
+    float model_inv[16];
+    float light_vp[16];
+    float view_pos[4];
+    float light_dir[4];
+    float light_color[4];
+    float ambient[4];
+    float mat_color[4];
+    float mat_spec[4];
+    float gloss;
+    float bump_scale;
+    int32_t enable_shadow;
+    int32_t pad_;
+};
+
+// One pass = one graphics_draw_triangle call over a mesh into a target.
+struct PassDesc {
+    // mesh (structure of arrays)
+    const float* px; const float* py; const float* pz;
+    const float* nx; const float* ny; const float* nz;
+    const float* tu; const float* tv;
+    const int32_t* idx;          // 3 per face
+    int32_t nverts, nfaces;
+    int32_t kind, nvary;
+    const DevUniforms* u;
+    // textures: RGBA8 words in TGAColor byte order
+    const uint32_t* diffuse; int32_t dw, dh;
+    const uint32_t* normal;  int32_t nw, nh;
+    // shadow map sampled by the fragment program (RGBA8 target words)
+    const uint32_t* shadow; int32_t sw, sh;
+    // target
+    uint32_t* color; float* depth; uint32_t* winner;
+    int32_t width, height, tiles_x, tiles_y;
+    int32_t clear;               // 1: the target is cleared to clear_rgba/clear_depth first
+    uint32_t clear_rgba; float clear_depth; int32_t pad0_;
+    // workspace slices owned by this pass
+    float4* vr;                  // per vertex: ndc.x, ndc.y, ndc.z, 1/w
+    float* vs;                   // per vertex: nvary varyings
+    uint64_t* vis;               // per 64 vertices: trivially-accept bits
+    int32_t* poly_of_face;       // clipped polygon of a face (-1: none)
+    uint64_t* stats;             // 5 counters (hana_pass_stats order), or null
+    // first block of this pass in each kernel's grid / first global tile
+    int32_t vblock0, fblock0, tile0, tblock0;
+};
+
+// Buffers shared by every pass of one launch sequence.
+struct Workspace {
+    uint32_t* tile_count;        // per global tile
+    uint32_t* tile_offset;       // exclusive scan of tile_count (+ total at [ntiles])
+    uint32_t* tile_cursor;
+    uint32_t* bins;              // triangle entries grouped by tile
+    float4* pool_vr;             // clipped polygons: POLY_MAX vertices each
+    float* pool_vs;              //   ... POLY_VARY varyings per vertex
+    int32_t* pool_face;          //   face (within its pass) of each polygon
+    int32_t* pool_n;             //   vertex count of each polygon
+    uint32_t* status;            // [0] poly count, [1] overflow flags, [2] bins needed, [3] polys needed
+    int32_t ntiles;
+    uint32_t bin_cap;
+    uint32_t poly_cap;
+    int32_t pad_;
+};
+
+enum { STATUS_POLYS = 0, STATUS_FLAGS = 1, STATUS_BINS_NEEDED = 2, STATUS_POLYS_NEEDED = 3, STATUS_WORDS = 4 };
+enum { FLAG_BIN_OVERFLOW = 1, FLAG_POLY_OVERFLOW = 2, FLAG_POLY_TOO_LARGE = 4 };
+enum { ST_FACES = 0, ST_CLIPPED = 1, ST_POLYS = 2, ST_ENTRIES = 3, ST_PIXELS = 4, ST_WORDS = 5 };
+
+}  // namespace hana

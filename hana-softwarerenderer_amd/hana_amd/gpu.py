@@ -1,0 +1,293 @@
+"""ctypes binding of libhana_gpu.so (include/hana_gpu.h).
+
+The product path is the HIP library only: if libhana_gpu.so is missing or no
+GPU is visible, loading fails with an exception -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libhana_gpu.so")
+
+HANA_OK, HANA_E_INVALID, HANA_E_DEVICE, HANA_E_NOMEM, HANA_E_OVERFLOW = 0, -1, -2, -3, -4
+GROUND, TOON, TEXTURE, TEXTURE_LIGHT, BLINN, NORMALMAP, SHADOW = range(7)
+FLT_MAX = float(np.finfo(np.float32).max)
+NO_WINNER = 0xFFFFFFFF
+
+# C declarations, in header order: (name, restype, argtypes)
+_P = C.c_void_p
+_FP = C.POINTER(C.c_float)
+
+
+class Uniforms(C.Structure):
+    _fields_ = [("model", C.c_float * 16), ("model_inv", C.c_float * 16), ("view", C.c_float * 16),
+                ("proj", C.c_float * 16), ("light_vp", C.c_float * 16), ("camera_vp", C.c_float * 16),
+                ("view_pos", C.c_float * 3), ("light_dir", C.c_float * 3), ("light_color", C.c_float * 4),
+                ("ambient", C.c_float * 4), ("mat_color", C.c_float * 4), ("mat_specular", C.c_float * 4),
+                ("gloss", C.c_float), ("bump_scale", C.c_float), ("enable_shadow", C.c_int32),
+                ("pad_", C.c_int32)]
+
+
+class DrawDesc(C.Structure):
+    _fields_ = [("mesh", _P), ("shader", C.c_int32), ("pad_", C.c_int32), ("uniforms", C.POINTER(Uniforms)),
+                ("diffuse", _P), ("normal", _P), ("shadow_map", _P), ("target", _P)]
+
+
+class FrameDesc(C.Structure):
+    _fields_ = [("mesh", _P), ("shader", C.c_int32), ("enable_shadow", C.c_int32),
+                ("uniforms", C.POINTER(Uniforms)), ("diffuse", _P), ("normal", _P), ("shadow_map", _P),
+                ("target", _P)]
+
+
+class PassStats(C.Structure):
+    _fields_ = [("faces", C.c_int64), ("clipped_faces", C.c_int64), ("polygons", C.c_int64),
+                ("tile_entries", C.c_int64), ("pixels_written", C.c_int64)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+DECLS = [
+    ("hana_gpu_create", C.c_int, [C.c_int, C.POINTER(_P)]),
+    ("hana_gpu_destroy", C.c_int, [_P]),
+    ("hana_gpu_last_error", C.c_char_p, [_P]),
+    ("hana_gpu_sync", C.c_int, [_P]),
+    ("hana_gpu_set_stats", C.c_int, [_P, C.c_int]),
+    ("hana_gpu_last_stats", C.c_int, [_P, C.POINTER(PassStats), C.c_int, C.POINTER(C.c_int)]),
+    ("hana_mesh_create", C.c_int, [_P, C.c_int32, _FP, _FP, _FP, C.c_int32, C.POINTER(C.c_int32), C.POINTER(_P)]),
+    ("hana_mesh_destroy", C.c_int, [_P, _P]),
+    ("hana_texture_create", C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_uint8), C.POINTER(_P)]),
+    ("hana_texture_destroy", C.c_int, [_P, _P]),
+    ("hana_target_create", C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(_P)]),
+    ("hana_target_destroy", C.c_int, [_P, _P]),
+    ("hana_target_clear", C.c_int, [_P, _P, C.POINTER(C.c_uint8), C.c_float]),
+    ("hana_target_upload", C.c_int, [_P, _P, C.POINTER(C.c_uint8), _FP]),
+    ("hana_target_read", C.c_int, [_P, _P, C.POINTER(C.c_uint8), _FP]),
+    ("hana_target_enable_winner", C.c_int, [_P, _P, C.c_int]),
+    ("hana_target_read_winner", C.c_int, [_P, _P, C.POINTER(C.c_uint32)]),
+    ("hana_draw", C.c_int, [_P, C.POINTER(DrawDesc)]),
+    ("hana_render_frame", C.c_int, [_P, C.POINTER(FrameDesc)]),
+    ("hana_batch_create", C.c_int, [_P, C.POINTER(_P)]),
+    ("hana_batch_add_frame", C.c_int, [_P, _P, C.POINTER(FrameDesc)]),
+    ("hana_batch_finalize", C.c_int, [_P, _P, C.c_int]),
+    ("hana_batch_run", C.c_int, [_P, _P]),
+    ("hana_batch_destroy", C.c_int, [_P, _P]),
+    ("hana_build_uniforms", C.c_int, [_FP, _FP, C.c_float, _FP, _FP, _FP, _FP, C.c_int32, C.c_int32, C.c_int32,
+                                      C.POINTER(Uniforms)]),
+]
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Loads libhana_gpu.so and declares every entry point (raises if absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built; run __graft_entry__.build() (no CPU fallback exists)")
+    lib = C.CDLL(path)
+    for name, res, args in DECLS:
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+class HanaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"hana error {code}: {msg}")
+        self.code = code
+
+
+def _f32(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if n is not None:
+        assert a.size == n, (a.size, n)
+    return a
+
+
+def _fp(a):
+    return a.ctypes.data_as(_FP)
+
+
+def build_uniforms(scene, enable_shadow=None) -> Uniforms:
+    """DrawModel::draw uniforms computed by the library's host math."""
+    lib = load_library()
+    u = Uniforms()
+    arrs = [_f32(x) for x in (scene.cam_pos, scene.cam_target, scene.light_pos, scene.obj_position,
+                              scene.obj_rotation, scene.obj_scale)]
+    es = scene.shadow if enable_shadow is None else enable_shadow
+    aspect = np.float32(scene.width) / np.float32(scene.height)
+    rc = lib.hana_build_uniforms(_fp(arrs[0]), _fp(arrs[1]), float(aspect), _fp(arrs[2]), _fp(arrs[3]), _fp(arrs[4]),
+                                 _fp(arrs[5]), scene.width, scene.height, int(es), C.byref(u))
+    if rc:
+        raise HanaError(rc, "hana_build_uniforms")
+    return u
+
+
+class Context:
+    """One HIP stream and the device objects created on it."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = _P()
+        rc = self.lib.hana_gpu_create(device, C.byref(h))
+        if rc:
+            raise HanaError(rc, "hana_gpu_create failed (is a GPU visible?)")
+        self.h = h
+        self._objs = []
+
+    def check(self, rc, what=""):
+        if rc:
+            msg = self.lib.hana_gpu_last_error(self.h)
+            raise HanaError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self.h:
+            for kind, o in reversed(self._objs):
+                getattr(self.lib, f"hana_{kind}_destroy")(self.h, o)
+            self._objs.clear()
+            self.lib.hana_gpu_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sync(self):
+        self.check(self.lib.hana_gpu_sync(self.h), "sync")
+
+    def set_stats(self, on: bool):
+        self.check(self.lib.hana_gpu_set_stats(self.h, int(on)))
+
+    def last_stats(self, max_passes=8):
+        arr = (PassStats * max_passes)()
+        n = C.c_int()
+        self.check(self.lib.hana_gpu_last_stats(self.h, arr, max_passes, C.byref(n)))
+        return [arr[i].as_dict() for i in range(n.value)]
+
+    # -- resources --
+    def mesh(self, m):
+        pos, nrm, uv = _f32(m.pos), _f32(m.nrm), _f32(m.uv)
+        idx = np.ascontiguousarray(m.idx, dtype=np.int32)
+        h = _P()
+        self.check(self.lib.hana_mesh_create(self.h, int(pos.shape[0]), _fp(pos), _fp(nrm), _fp(uv), int(idx.shape[0]),
+                                             idx.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(h)), "mesh")
+        self._objs.append(("mesh", h))
+        return h
+
+    def texture(self, img: np.ndarray):
+        img = np.ascontiguousarray(img, dtype=np.uint8)
+        hgt, w = img.shape[:2]
+        bpp = 1 if img.ndim == 2 else img.shape[2]
+        h = _P()
+        self.check(self.lib.hana_texture_create(self.h, w, hgt, bpp, img.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                                C.byref(h)), "texture")
+        self._objs.append(("texture", h))
+        return h
+
+    def target(self, w, h, rgba=(0, 0, 0, 255), depth=FLT_MAX, winner=False):
+        t = _P()
+        self.check(self.lib.hana_target_create(self.h, int(w), int(h), C.byref(t)), "target")
+        self._objs.append(("target", t))
+        if rgba is not None:
+            self.clear(t, rgba, depth)
+        if winner:
+            self.check(self.lib.hana_target_enable_winner(self.h, t, 1))
+        return t
+
+    def clear(self, t, rgba=(0, 0, 0, 255), depth=FLT_MAX):
+        b = (C.c_uint8 * 4)(*rgba)
+        self.check(self.lib.hana_target_clear(self.h, t, b, float(depth)), "clear")
+
+    def upload(self, t, color: np.ndarray | None, depth: np.ndarray | None):
+        cp = np.ascontiguousarray(color, np.uint8).ctypes.data_as(C.POINTER(C.c_uint8)) if color is not None else None
+        dp = _fp(np.ascontiguousarray(depth, np.float32)) if depth is not None else None
+        self.check(self.lib.hana_target_upload(self.h, t, cp, dp), "upload")
+
+    def read(self, t, w, h):
+        color = np.zeros((h, w, 4), np.uint8)
+        depth = np.zeros((h, w), np.float32)
+        self.check(self.lib.hana_target_read(self.h, t, color.ctypes.data_as(C.POINTER(C.c_uint8)), _fp(depth)), "read")
+        return color, depth
+
+    def read_winner(self, t, w, h):
+        win = np.zeros((h, w), np.uint32)
+        self.check(self.lib.hana_target_read_winner(self.h, t, win.ctypes.data_as(C.POINTER(C.c_uint32))), "winner")
+        return win
+
+    # -- drawing --
+    def draw(self, mesh, kind, u: Uniforms, target, diffuse=None, normal=None, shadow_map=None):
+        d = DrawDesc(mesh, kind, 0, C.pointer(u), diffuse, normal, shadow_map, target)
+        self.check(self.lib.hana_draw(self.h, C.byref(d)), "draw")
+
+    def frame_desc(self, mesh, kind, shadow, u: Uniforms, target, shadow_map=None, diffuse=None, normal=None):
+        return FrameDesc(mesh, kind, int(shadow), C.pointer(u), diffuse, normal, shadow_map, target)
+
+    def render_frame(self, mesh, kind, shadow, u: Uniforms, target, shadow_map=None, diffuse=None, normal=None):
+        d = self.frame_desc(mesh, kind, shadow, u, target, shadow_map, diffuse, normal)
+        self.check(self.lib.hana_render_frame(self.h, C.byref(d)), "render_frame")
+
+
+class Batch:
+    """Many frames rendered by one launch sequence (optionally a hipGraph)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        h = _P()
+        ctx.check(ctx.lib.hana_batch_create(ctx.h, C.byref(h)))
+        self.h = h
+        self._keep = []
+
+    def add(self, desc: FrameDesc):
+        self._keep.append(desc)
+        self.ctx.check(self.ctx.lib.hana_batch_add_frame(self.ctx.h, self.h, C.byref(desc)), "batch_add")
+
+    def finalize(self, use_graph=True):
+        self.ctx.check(self.ctx.lib.hana_batch_finalize(self.ctx.h, self.h, int(use_graph)), "batch_finalize")
+
+    def run(self):
+        self.ctx.check(self.ctx.lib.hana_batch_run(self.ctx.h, self.h), "batch_run")
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.hana_batch_destroy(self.ctx.h, self.h)
+            self.h = None
+
+
+class SceneOnDevice:
+    """A generated scene's mesh, textures, targets and uniforms on one context."""
+
+    def __init__(self, ctx: Context, scene, winner=False):
+        self.ctx, self.scene = ctx, scene
+        self.mesh = ctx.mesh(scene.mesh)
+        self.diffuse = ctx.texture(scene.diffuse) if scene.diffuse is not None else None
+        self.normal = ctx.texture(scene.normal) if scene.normal is not None else None
+        self.target = ctx.target(scene.width, scene.height, winner=winner)
+        self.shadow_map = None
+        if scene.shadow:
+            sw, sh = scene.shadow_wh
+            self.shadow_map = ctx.target(sw, sh)
+        self.u = build_uniforms(scene)
+
+    def desc(self) -> FrameDesc:
+        return self.ctx.frame_desc(self.mesh, self.scene.shader, self.scene.shadow, self.u, self.target,
+                                   self.shadow_map, self.diffuse, self.normal)
+
+    def render(self):
+        self.ctx.render_frame(self.mesh, self.scene.shader, self.scene.shadow, self.u, self.target, self.shadow_map,
+                              self.diffuse, self.normal)
+
+    def clear(self):
+        self.ctx.clear(self.target)
+
+    def read(self):
+        return self.ctx.read(self.target, self.scene.width, self.scene.height)

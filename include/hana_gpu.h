@@ -1,0 +1,170 @@
+/*
+ * hana_gpu.h -- C ABI of the MI355X (gfx950) rasterisation pipeline.
+ *
+ * Plain C types only: pointers, sizes, PODs.  Every function returns 0 on
+ * success and a negative HANA_E* code on failure; hana_gpu_last_error()
+ * describes the last failure on a context.  No exception crosses this ABI.
+ *
+ * Each entry point names the reference interface it replaces
+ * (SR/x:N = Hana-SoftwareRenderer/x line N of the reference):
+ *
+ *   hana_draw            graphics_draw_triangle(DrawData*)         SR/graphics.h:15, SR/graphics.cpp:378-407
+ *   hana_render_frame    DrawModel::draw(Camera*, RenderBuffer*, bool)  SR/scene.h:53-99
+ *   hana_mesh_create     Model accessors vert/normal/uv             SR/model.h:19-33, SR/model.cpp:66-111
+ *   hana_texture_create  TGAImage::get / TGAColor(p, bpp)           SR/tgaimage.h:46-53,66-98, SR/tgaimage.cpp:248-253
+ *   hana_target_*        RenderBuffer ctor / clears / buffers       SR/renderbuffer.h:5-21, SR/renderbuffer.cpp:3-76
+ *   hana_uniforms        ShaderData + Material (no pointers)        SR/IShader.h:7-32
+ *   hana_build_uniforms  DrawModel::draw uniform set-up             SR/scene.h:55-71, SR/camera.cpp:82-92
+ *   HANA_SHADER_*        the IShader subclasses                     SR/IShader.h:132-165
+ *
+ * A context owns one HIP stream and every device object created on it; calls
+ * on one context must be externally synchronised.  Work is enqueued
+ * asynchronously; hana_target_read and hana_gpu_sync wait for it.
+ */
+#ifndef HANA_GPU_H
+#define HANA_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    HANA_OK = 0,
+    HANA_E_INVALID = -1,     /* bad argument */
+    HANA_E_DEVICE = -2,      /* HIP runtime error */
+    HANA_E_NOMEM = -3,       /* device allocation failed or over budget */
+    HANA_E_OVERFLOW = -4     /* internal capacity exceeded after retry */
+};
+
+/* IShader subclasses (SR/IShader.h:132-165). */
+enum {
+    HANA_SHADER_GROUND = 0,
+    HANA_SHADER_TOON = 1,
+    HANA_SHADER_TEXTURE = 2,
+    HANA_SHADER_TEXTURE_LIGHT = 3,
+    HANA_SHADER_BLINN = 4,
+    HANA_SHADER_NORMALMAP = 5,
+    HANA_SHADER_SHADOW = 6
+};
+
+typedef struct hana_ctx hana_ctx;
+typedef struct hana_mesh hana_mesh;
+typedef struct hana_texture hana_texture;
+typedef struct hana_target hana_target;
+typedef struct hana_batch hana_batch;
+
+/* ShaderData + Material scalars (SR/IShader.h:7-32); row-major 4x4. */
+typedef struct hana_uniforms {
+    float model[16];
+    float model_inv[16];
+    float view[16];
+    float proj[16];
+    float light_vp[16];
+    float camera_vp[16];
+    float view_pos[3];
+    float light_dir[3];
+    float light_color[4];
+    float ambient[4];
+    float mat_color[4];
+    float mat_specular[4];
+    float gloss;
+    float bump_scale;
+    int32_t enable_shadow;
+    int32_t pad_;
+} hana_uniforms;
+
+/* One graphics_draw_triangle call (DrawData, SR/graphics.h:9-13). */
+typedef struct hana_draw_desc {
+    hana_mesh* mesh;
+    int32_t shader;                 /* HANA_SHADER_* */
+    int32_t pad_;
+    const hana_uniforms* uniforms;
+    hana_texture* diffuse;          /* Material::diffuse_map, may be NULL */
+    hana_texture* normal;           /* Material::normal_map, may be NULL */
+    hana_target* shadow_map;        /* ShaderData::shadow_map, may be NULL */
+    hana_target* target;            /* DrawData::render_buffer */
+} hana_draw_desc;
+
+/* One DrawModel::draw call (SR/scene.h:53-99). */
+typedef struct hana_frame_desc {
+    hana_mesh* mesh;
+    int32_t shader;
+    int32_t enable_shadow;
+    const hana_uniforms* uniforms;
+    hana_texture* diffuse;
+    hana_texture* normal;
+    hana_target* shadow_map;        /* required when enable_shadow */
+    hana_target* target;
+} hana_frame_desc;
+
+/* Per-pass counters (filled when stats are enabled on the context). */
+typedef struct hana_pass_stats {
+    int64_t faces;           /* faces submitted */
+    int64_t clipped_faces;   /* faces failing the trivial-accept test */
+    int64_t polygons;        /* clipped polygons with >= 3 vertices */
+    int64_t tile_entries;    /* (triangle, tile) bin entries */
+    int64_t pixels_written;  /* pixels whose colour/depth the pass wrote */
+} hana_pass_stats;
+
+/* ---- context ---- */
+int hana_gpu_create(int device, hana_ctx** out);
+int hana_gpu_destroy(hana_ctx* ctx);
+const char* hana_gpu_last_error(hana_ctx* ctx);
+int hana_gpu_sync(hana_ctx* ctx);
+/* Collect hana_pass_stats for subsequent draws (adds a few atomics). */
+int hana_gpu_set_stats(hana_ctx* ctx, int enable);
+int hana_gpu_last_stats(hana_ctx* ctx, hana_pass_stats* out, int max_passes, int* n_passes);
+
+/* ---- resources ---- */
+/* SoA fp32 attributes (pos 3V, nrm 3V, uv 2V) and int32 indices (3F); one
+ * index per corner for position, normal and uv.  Normals are used as
+ * Model::normal returns them: renormalised (SR/model.cpp:108-111). */
+int hana_mesh_create(hana_ctx* ctx, int32_t nverts, const float* pos, const float* nrm,
+                     const float* uv, int32_t nfaces, const int32_t* idx, hana_mesh** out);
+int hana_mesh_destroy(hana_ctx* ctx, hana_mesh* m);
+/* TGAImage bytes: bpp in {1,3,4}, B,G,R,A order, texel (x,y) at (x+y*w)*bpp. */
+int hana_texture_create(hana_ctx* ctx, int32_t w, int32_t h, int32_t bpp, const uint8_t* bytes,
+                        hana_texture** out);
+int hana_texture_destroy(hana_ctx* ctx, hana_texture* t);
+/* RenderBuffer(width, height): RGBA8 at (y*W+x)*4 and fp32 depth at y*W+x,
+ * initialised like the reference constructor (colour 0,0,0,255, depth 1). */
+int hana_target_create(hana_ctx* ctx, int32_t w, int32_t h, hana_target** out);
+int hana_target_destroy(hana_ctx* ctx, hana_target* t);
+/* renderbuffer_clear_color + renderbuffer_clear_depth; deferred and fused
+ * into the next pass that draws into the target. rgba bytes as written. */
+int hana_target_clear(hana_ctx* ctx, hana_target* t, const uint8_t rgba[4], float depth);
+int hana_target_upload(hana_ctx* ctx, hana_target* t, const uint8_t* rgba, const float* depth);
+int hana_target_read(hana_ctx* ctx, hana_target* t, uint8_t* rgba, float* depth);
+/* Optional per-pixel winner map: for each written pixel the primitive id
+ * face*16 + (trivially accepted ? 0 : 1 + fan index), 0xFFFFFFFF elsewhere.
+ * Enabled until the target is destroyed; read with hana_target_read_winner. */
+int hana_target_enable_winner(hana_ctx* ctx, hana_target* t, int enable);
+int hana_target_read_winner(hana_ctx* ctx, hana_target* t, uint32_t* winner);
+
+/* ---- drawing ---- */
+int hana_draw(hana_ctx* ctx, const hana_draw_desc* d);
+int hana_render_frame(hana_ctx* ctx, const hana_frame_desc* f);
+
+/* ---- batches: many frames in one captured graph ---- */
+int hana_batch_create(hana_ctx* ctx, hana_batch** out);
+int hana_batch_add_frame(hana_ctx* ctx, hana_batch* b, const hana_frame_desc* f);
+/* Build the workspace and (when use_graph) capture the launch sequence. */
+int hana_batch_finalize(hana_ctx* ctx, hana_batch* b, int use_graph);
+int hana_batch_run(hana_ctx* ctx, hana_batch* b);
+int hana_batch_destroy(hana_ctx* ctx, hana_batch* b);
+
+/* ---- host uniform math ---- */
+/* ShaderData as DrawModel::draw fills it for Camera(cam_pos, cam_target,
+ * cam_aspect), a light GameObject at light_pos, the model transform
+ * (position, rotation in degrees, scale) and a framebuffer fb_w x fb_h. */
+int hana_build_uniforms(const float cam_pos[3], const float cam_target[3], float cam_aspect,
+                        const float light_pos[3], const float obj_position[3],
+                        const float obj_rotation_deg[3], const float obj_scale[3], int32_t fb_w,
+                        int32_t fb_h, int32_t enable_shadow, hana_uniforms* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,145 @@
+"""GPU pipeline against the oracle on the same seeded inputs (through the C ABI).
+
+Bar: depth and the winning primitive of every pixel bit-exact; colour within
+1 LSB on >= 99.9 % of pixels and 2 LSB everywhere; the shadow map (R8 of the
+interpolated light-space depth) bit-exact.
+"""
+import numpy as np
+import pytest
+
+from hana_amd import scenegen as sg
+from parity import assert_color_ok, assert_depth_exact
+from pyoracle import CPURenderer, HostTarget
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from hana_amd.gpu import Context
+    c = Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def oracle():
+    return CPURenderer("oracle")
+
+
+def gpu_passes(ctx, scene, winner=True):
+    """Shadow pass then main pass as two hana_draw calls; returns target and shadow map."""
+    from hana_amd.gpu import SHADOW, build_uniforms
+    u = build_uniforms(scene)
+    mesh = ctx.mesh(scene.mesh)
+    dif = ctx.texture(scene.diffuse)
+    nrm = ctx.texture(scene.normal)
+    tgt = ctx.target(scene.width, scene.height, winner=winner)
+    sm = None
+    smap = None
+    if scene.shadow:
+        sw, sh = scene.shadow_wh
+        sm = ctx.target(sw, sh, winner=winner)
+        ctx.draw(mesh, SHADOW, u, sm, dif, nrm)
+    ctx.draw(mesh, scene.shader, u, tgt, dif, nrm, sm)
+    col, dep = ctx.read(tgt, scene.width, scene.height)
+    win = ctx.read_winner(tgt, scene.width, scene.height) if winner else None
+    if sm is not None:
+        sw, sh = scene.shadow_wh
+        sc, sd = ctx.read(sm, sw, sh)
+        swin = ctx.read_winner(sm, sw, sh) if winner else None
+        smap = (sc, sd, swin)
+    return col, dep, win, smap
+
+
+def oracle_passes(oracle, scene):
+    u = oracle.build_uniforms(scene)
+    tgt = HostTarget(scene.width, scene.height)
+    sm = None
+    st_s = None
+    if scene.shadow:
+        sw, sh = scene.shadow_wh
+        sm = HostTarget(sw, sh)
+        st_s, swin = oracle.draw(scene.mesh, sg.SHADOW, u, sm, scene.diffuse, scene.normal, winner=True)
+        sm.winner = swin
+    st, win = oracle.draw(scene.mesh, scene.shader, u, tgt, scene.diffuse, scene.normal, sm, winner=True)
+    return tgt, win, sm, st, st_s
+
+
+def check_scene(ctx, oracle, scene):
+    col, dep, win, smap = gpu_passes(ctx, scene)
+    tgt, owin, osm, st, st_s = oracle_passes(oracle, scene)
+    if scene.shadow:
+        sc, sd, swin = smap
+        assert_depth_exact(sd, osm.depth, scene.name + " shadow depth")
+        assert np.array_equal(swin.astype(np.int64), np.where(osm.winner < 0, 0xFFFFFFFF, osm.winner).astype(np.int64)), \
+            scene.name + " shadow coverage"
+        assert np.array_equal(sc[..., :3], osm.color[..., :3]), scene.name + " shadow colour"
+    owin_u = np.where(owin < 0, 0xFFFFFFFF, owin).astype(np.int64)
+    assert np.array_equal(win.astype(np.int64), owin_u), \
+        f"{scene.name}: coverage differs at {int((win.astype(np.int64) != owin_u).sum())} pixels"
+    assert_depth_exact(dep, tgt.depth, scene.name)
+    rep = assert_color_ok(col, tgt.color, scene.name)
+    return rep, st
+
+
+SHADERS = [sg.GROUND, sg.TOON, sg.TEXTURE, sg.TEXTURE_LIGHT, sg.BLINN, sg.NORMALMAP]
+
+
+@pytest.mark.parametrize("kind", SHADERS)
+def test_small_each_shader(ctx, oracle, kind):
+    scene = sg.small_scene(kind, shadow=False, seed=10 + kind)
+    check_scene(ctx, oracle, scene)
+
+
+@pytest.mark.parametrize("kind", [sg.BLINN, sg.NORMALMAP])
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_small_shadowed(ctx, oracle, kind, seed):
+    scene = sg.small_scene(kind, shadow=True, seed=seed)
+    check_scene(ctx, oracle, scene)
+
+
+@pytest.mark.parametrize("cam", [(0.3, 0.2, 1.2), (0.0, 0.0, 1.05), (0.9, -0.4, 0.6), (0.0, 1.1, 0.2)])
+def test_small_clipping(ctx, oracle, cam):
+    """Cameras close to or inside the mesh: near-plane and screen-edge clipping."""
+    scene = sg.small_scene(sg.NORMALMAP, shadow=True, seed=7, cam=cam, width=128, height=96)
+    check_scene(ctx, oracle, scene)
+
+
+@pytest.mark.parametrize("wh", [(1, 1), (7, 5), (33, 31), (65, 17), (200, 37)])
+def test_ragged_sizes(ctx, oracle, wh):
+    scene = sg.small_scene(sg.BLINN, shadow=True, seed=5, width=wh[0], height=wh[1])
+    check_scene(ctx, oracle, scene)
+
+
+def test_config1(ctx, oracle):
+    for kind in (sg.NORMALMAP, sg.BLINN):
+        scene = sg.config1_scene(kind)
+        check_scene(ctx, oracle, scene)
+
+
+def test_render_frame_semantics(ctx, oracle):
+    """DrawModel::draw: shadow pass, main pass, shadow map cleared afterwards;
+    two frames in a row give the same image."""
+    from hana_amd.gpu import SceneOnDevice
+    scene = sg.small_scene(sg.NORMALMAP, shadow=True, seed=21)
+    dev = SceneOnDevice(ctx, scene)
+    dev.render()
+    col1, dep1 = dev.read()
+    sw, sh = scene.shadow_wh
+    sc, sd = ctx.read(dev.shadow_map, sw, sh)
+    assert (sc[..., :3] == 0).all() and (sd == np.float32(3.4028234663852886e38)).all()
+    dev.clear()
+    dev.render()
+    col2, dep2 = dev.read()
+    assert np.array_equal(col1, col2) and np.array_equal(dep1.view(np.uint32), dep2.view(np.uint32))
+    tgt, st_s, st_m = oracle.render_frame(scene)
+    assert_depth_exact(dep1, tgt.depth, "frame")
+    assert_color_ok(col1, tgt.color, "frame")
+
+
+@pytest.mark.slow
+def test_config2_full(ctx, oracle):
+    scene = sg.config2_scene()
+    rep, st = check_scene(ctx, oracle, scene)
+    assert st["fragments"] > 1_000_000

@@ -1,0 +1,306 @@
+"""Pins the oracle (oracle/hana_oracle.c) before it is trusted as the checker.
+
+1. Golden vectors written by the reference's own code (tests/golden/*.npz,
+   made by tests/golden/make_golden.py from oracle/_ref): bit-exact.
+2. The live compiled reference (oracle/_ref/libhanaref.so) where it is built:
+   bit-exact on fresh seeded inputs, whole frames included.
+3. Closed forms and invariants that a plausible slip (dropped term, wrong
+   sign or index, transposed operand) would break.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from hana_amd import scenegen as sg
+from pyoracle import REF_SO, CPURenderer, HostTarget, Uniforms
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+HAVE_REF = os.path.exists(REF_SO)
+
+
+@pytest.fixture(scope="module")
+def oracle():
+    return CPURenderer("oracle")
+
+
+@pytest.fixture(scope="module")
+def ref():
+    if not HAVE_REF:
+        pytest.skip("oracle/_ref is not built (no reference tree in this environment)")
+    return CPURenderer("reference")
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def beq(a, b):
+    return np.array_equal(bits(a), bits(b))
+
+
+def load(name):
+    return np.load(os.path.join(GOLDEN, name))
+
+
+# ---------------------------------------------------------------- golden vectors
+
+def test_golden_math(oracle):
+    g = load("kat_math.npz")
+    for i in range(len(g["mm_a"])):
+        assert beq(oracle.mat4_mul(g["mm_a"][i], g["mm_b"][i]), g["mm_out"][i])
+        assert beq(oracle.mat4_mul_vec4(g["mm_a"][i], g["mv_v"][i]), g["mv_out"][i])
+    for i in range(len(g["inv_in"])):
+        assert beq(oracle.invert(g["inv_in"][i]), g["inv_out"][i]), i
+    up = np.array([0, 1, 0], np.float32)
+    for i in range(len(g["look_eye"])):
+        assert beq(oracle.lookat(g["look_eye"][i], g["look_tgt"][i], up), g["look_out"][i])
+    for i, p in enumerate(g["persp_in"]):
+        assert beq(oracle.perspective(*p), g["persp_out"][i])
+        assert beq(oracle.orthographic(p[1], 1.0, 0.0, p[3]), g["ortho_out"][i])
+    for i, t in enumerate(g["rot_in"]):
+        assert beq(oracle.rotate_axis("x", t), g["rotx_out"][i])
+        assert beq(oracle.rotate_axis("y", t), g["roty_out"][i])
+        assert beq(oracle.rotate_axis("z", t), g["rotz_out"][i])
+        assert beq(oracle.rotate(t, *g["rot_axis"][i]), g["rot_out"][i])
+    for i in range(len(g["model_p"])):
+        assert beq(oracle.model_matrix(g["model_p"][i], g["model_r"][i], g["model_s"][i]), g["model_out"][i])
+    for i in range(len(g["vp_ndc"])):
+        w, h = g["vp_wh"][i]
+        assert beq(oracle.viewport(int(w), int(h), g["vp_ndc"][i]), g["vp_out"][i])
+
+
+def test_golden_uniforms(oracle):
+    g = load("kat_math.npz")
+    scenes = [sg.small_scene(sg.BLINN, seed=s) for s in range(4)] + [sg.config1_scene()]
+    for i, sc in enumerate(scenes):
+        assert np.array_equal(np.array(sc.cam_pos, np.float32), g["u_cam"][i])
+        u = oracle.build_uniforms(sc)
+        assert bytes(u) == g["u_bytes"][i].tobytes(), sc.name
+
+
+def test_golden_raster(oracle):
+    g = load("kat_raster.npz")
+    for i in range(len(g["bary_tri"])):
+        t = g["bary_tri"][i]
+        assert beq(oracle.barycentric(t[0], t[1], t[2], g["bary_p"][i]), g["bary_out"][i]), i
+    for i, x in enumerate(g["bf_ndc"]):
+        assert int(oracle.is_back_facing(x)) == g["bf_out"][i]
+    for i, c in enumerate(g["clip_in"]):
+        r = oracle.clip_triangle(c)
+        assert r.shape[0] == g["clip_n"][i], i
+        assert beq(r, g["clip_out"][i, : r.shape[0]]), i
+
+
+def test_golden_shaders(oracle):
+    g = load("kat_shaders.npz")
+    sc = sg.small_scene(sg.NORMALMAP, shadow=True, seed=77, width=48, height=40, tex=32)
+    u = oracle.build_uniforms(sc)
+    assert bytes(u) == g["u_bytes"].tobytes()
+    sm = HostTarget(48, 40)
+    sm.color[...] = g["shadow_color"]
+    for kind in range(7):
+        vo = np.stack([oracle.vertex(kind, u, a) for a in g["a2v"]])
+        assert beq(vo, g[f"vert_{kind}"]), kind
+        for shadow in (0, 1):
+            uu = oracle.build_uniforms(sc, enable_shadow=shadow)
+            fo = np.stack([oracle.fragment(kind, uu, v, g["diffuse"], g["normal"], sm) for v in g["v2f"]])
+            assert beq(fo, g[f"frag_{kind}_{shadow}"]), (kind, shadow)
+
+
+FRAMES = ["blinn", "normalmap", "shadow_nm", "edge_clip", "near_clip", "toon", "texture_light"]
+
+
+@pytest.mark.parametrize("name", FRAMES)
+def test_golden_frames(oracle, name):
+    g = load("frames.npz")
+    k = lambda s: g[f"{name}__{s}"]
+    w, h, shader, shadow = (int(x) for x in k("meta"))
+    mesh = sg.Mesh(k("pos"), k("nrm"), k("uv"), k("idx"))
+    sc = sg.Scene(mesh, w, h, shader, bool(shadow), tuple(float(x) for x in k("cam")),
+                  tuple(float(x) for x in k("light")), k("diffuse"), k("normal"), name=name)
+    tgt, sm, st_s, st_m = oracle.render_passes(sc)
+    assert np.array_equal(tgt.color[..., :3], k("color")[..., :3])
+    assert beq(tgt.depth, k("depth"))
+    assert st_m["fragments"] == int(k("fragments"))
+    if shadow:
+        assert np.array_equal(sm.color[..., :3], k("shadow_color")[..., :3])
+        assert beq(sm.depth, k("shadow_depth"))
+        assert st_s["fragments"] == int(k("shadow_fragments"))
+
+
+# ---------------------------------------------------------------- live reference
+
+@pytest.mark.parametrize("seed", range(6))
+def test_live_frames_match_reference(oracle, ref, seed):
+    kind = [sg.BLINN, sg.NORMALMAP, sg.TOON, sg.TEXTURE, sg.GROUND, sg.TEXTURE_LIGHT][seed]
+    cams = [(0.0, 0.0, 2.0), (0.5, 0.4, 1.4), (0.1, -0.2, 1.03), (1.2, 0.8, 1.1), (0.0, 1.5, 0.4), (-0.3, 0.2, 1.2)]
+    sc = sg.small_scene(kind, shadow=kind in (sg.BLINN, sg.NORMALMAP), seed=100 + seed, cam=cams[seed],
+                        width=96, height=72)
+    a = oracle.render_passes(sc)
+    b = ref.render_passes(sc)
+    assert np.array_equal(a[0].color[..., :3], b[0].color[..., :3])
+    assert beq(a[0].depth, b[0].depth)
+    assert a[3]["fragments"] == b[3]["fragments"]
+    assert a[3]["clipped_faces"] == b[3]["clipped_faces"]
+    if sc.shadow:
+        assert np.array_equal(a[1].color[..., :3], b[1].color[..., :3])
+        assert beq(a[1].depth, b[1].depth)
+
+
+def test_live_config1_matches_reference(oracle, ref):
+    sc = sg.config1_scene()
+    a = oracle.render_passes(sc)
+    b = ref.render_passes(sc)
+    assert np.array_equal(a[0].color[..., :3], b[0].color[..., :3]) and beq(a[0].depth, b[0].depth)
+    assert a[3]["fragments"] == b[3]["fragments"]
+
+
+def test_unmodified_entry_matches_driver(oracle, ref, tmp_path):
+    """graphics_draw_triangle itself (on a Model read by the reference's OBJ
+    parser) equals the capacity-fixed face loop on a scene with no clipping."""
+    import ctypes as C
+    from pyoracle import Image, Target, image_struct
+    sc = sg.small_scene(sg.BLINN, shadow=False, seed=5, width=80, height=60, cam=(0.0, 0.3, 3.0))
+    m = sc.mesh
+    path = tmp_path / "m.obj"
+    with open(path, "w") as f:
+        for p in m.pos:
+            f.write("v %.9g %.9g %.9g\n" % tuple(p))
+        for t in m.uv:
+            f.write("vt %.9g %.9g 0\n" % tuple(t))
+        for n in m.nrm:
+            f.write("vn %.9g %.9g %.9g\n" % tuple(n))
+        for a, b, c in m.idx + 1:
+            f.write(f"f {a}/{a}/{a} {b}/{b}/{b} {c}/{c}/{c}\n")
+    u = ref.build_uniforms(sc)
+    t1 = HostTarget(sc.width, sc.height)
+    d, n = image_struct(sc.diffuse), image_struct(sc.normal)
+    tt = t1.struct()
+    ref._draw_model(str(path).encode(), sg.BLINN, C.byref(u), C.byref(d), C.byref(n), None, C.byref(tt))
+    st, _ = oracle.draw(sc.mesh, sg.BLINN, oracle.build_uniforms(sc), t2 := HostTarget(sc.width, sc.height),
+                        sc.diffuse, sc.normal)
+    assert st["clipped_faces"] == 0
+    assert np.array_equal(t1.color[..., :3], t2.color[..., :3]) and beq(t1.depth, t2.depth)
+
+
+# ---------------------------------------------------------------- closed forms / invariants
+
+def test_viewport_closed_form(oracle):
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        ndc = rng.uniform(-1, 1, 3).astype(np.float32)
+        w, h = int(rng.integers(1, 5000)), int(rng.integers(1, 5000))
+        o = oracle.viewport(w, h, ndc)
+        f = np.float32
+        exp = [(ndc[0] + f(1)) * f(0.5) * f(w), (ndc[1] + f(1)) * f(0.5) * f(h), (ndc[2] + f(1)) * f(0.5)]
+        assert beq(o, np.array(exp, np.float32))
+    assert beq(oracle.viewport(800, 600, np.array([-1, -1, -1], np.float32)), np.zeros(3, np.float32))
+
+
+def test_identity_transforms(oracle):
+    eye = np.eye(4, dtype=np.float32)
+    assert np.array_equal(oracle.model_matrix([0, 0, 0], [0, 0, 0], [1, 1, 1]), eye)
+    for ax in "xyz":
+        assert np.array_equal(oracle.rotate_axis(ax, 0.0), eye)
+    m = oracle.model_matrix([0.5, -1, 2], [10, 20, 30], [1.5, 0.5, 2.0])
+    inv = oracle.invert(m)
+    assert np.allclose(inv @ m, eye, atol=1e-5)
+    a = np.arange(16, dtype=np.float32).reshape(4, 4)
+    b = np.eye(4, dtype=np.float32)[::-1].copy()
+    assert np.array_equal(oracle.mat4_mul(a, b), a @ b)        # permutation: exact, catches transposes
+    assert np.array_equal(oracle.mat4_mul(b, a), b @ a)
+
+
+def test_light_vp_w_is_one(oracle):
+    """orthographic * lookat has bottom row (0,0,0,1): shadow-pass w == 1 exactly."""
+    for s in range(8):
+        sc = sg.small_scene(sg.NORMALMAP, shadow=True, seed=s)
+        u = oracle.build_uniforms(sc)
+        lvp = np.array(u.light_vp, np.float32).reshape(4, 4)
+        assert np.array_equal(lvp[3], np.array([0, 0, 0, 1], np.float32))
+        v = oracle.vertex(sg.SHADOW, u, np.array([0.3, -0.2, 0.5, 0, 0, 1, 0.5, 0.5], np.float32))
+        assert v[3] == 1.0
+
+
+def test_barycentric_properties(oracle):
+    """Vertices get unit weights; points outside get a negative weight; a
+    degenerate triangle gives (-1, 1, 1)."""
+    A, B, C = np.array([2, 3], np.float32), np.array([20, 5], np.float32), np.array([6, 30], np.float32)
+    assert np.array_equal(oracle.barycentric(A, B, C, A), [1, 0, 0])
+    assert np.array_equal(oracle.barycentric(A, B, C, B), [0, 1, 0])
+    assert np.array_equal(oracle.barycentric(A, B, C, C), [0, 0, 1])
+    assert (oracle.barycentric(A, B, C, np.array([40, 40], np.float32)) < 0).any()
+    assert np.array_equal(oracle.barycentric(A, A, C, B), [-1, 1, 1])
+    # brute force: float decision equals the exact rational sign away from edges
+    rng = np.random.default_rng(2)
+    for _ in range(300):
+        t = rng.integers(0, 40, (3, 2)).astype(np.float32)
+        P = rng.integers(0, 40, 2).astype(np.float32)
+        (ax, ay), (bx, by), (cx, cy) = t.astype(np.int64)
+        px, py = P.astype(np.int64)
+        uz = (cx - ax) * (by - ay) - (bx - ax) * (cy - ay)
+        if uz == 0:
+            continue
+        ux = (bx - ax) * (ay - py) - (ax - px) * (by - ay)
+        uy = (ax - px) * (cy - ay) - (cx - ax) * (ay - py)
+        exact_in = (uy * uz >= 0) and (ux * uz >= 0) and ((uz - ux - uy) * uz >= 0)
+        w = oracle.barycentric(t[0], t[1], t[2], P)
+        assert (not (w < 0).any()) == exact_in
+
+
+def test_clip_invariants(oracle):
+    rng = np.random.default_rng(3)
+    for _ in range(300):
+        v = np.zeros((3, 13), np.float32)
+        v[:, 3] = rng.uniform(-0.5, 2.0, 3)
+        v[:, :3] = rng.uniform(-3, 3, (3, 3))
+        v[:, 4:] = rng.normal(size=(3, 9))
+        r = oracle.clip_triangle(v)
+        assert r.shape[0] == 0 or 3 <= r.shape[0] <= 10
+        if r.shape[0]:
+            w = r[:, 3]
+            assert (w >= 1e-5 * (1 - 1e-6)).all()
+            tol = 1e-4 * np.maximum(1.0, np.abs(w))
+            assert (np.abs(r[:, :3]) <= w[:, None] + tol[:, None]).all()
+    # a visible triangle is returned unchanged (trivial accept)
+    v = np.zeros((3, 13), np.float32)
+    v[:, 3] = 1.0
+    v[:, :2] = [[0, 0], [0.5, 0], [0, 0.5]]
+    v[:, 4:] = np.arange(27, dtype=np.float32).reshape(3, 9)
+    assert np.array_equal(oracle.clip_triangle(v), v)
+
+
+def test_depth_order_semantics(oracle):
+    """LEQUAL with the later primitive winning ties: two identical quads, the
+    second with other uvs, leave the second's colour; a nearer first quad wins."""
+    pos = np.array([[-0.5, -0.5, 0], [0.5, -0.5, 0], [0.5, 0.5, 0], [-0.5, 0.5, 0]], np.float32)
+    quad = np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+    uv1 = np.full((4, 2), 0.1, np.float32)
+    uv2 = np.full((4, 2), 0.9, np.float32)
+    nrm = np.tile(np.array([[0, 0, 1]], np.float32), (4, 1))
+    tex = np.zeros((2, 2, 4), np.uint8)
+    tex[0, 0] = (10, 20, 30, 255)
+    tex[1, 1] = (200, 100, 50, 255)
+    mesh = sg.Mesh(np.concatenate([pos, pos]), np.concatenate([nrm, nrm]), np.concatenate([uv1, uv2]),
+                   np.concatenate([quad, quad + 4]))
+    sc = sg.Scene(mesh, 32, 32, sg.TEXTURE, False, (0.0, 0.0, 2.0), (1.0, 1.0, 1.0), tex, tex)
+    tgt, _, _, st = oracle.render_passes(sc)
+    covered = tgt.depth < 1e30
+    assert covered.sum() > 50
+    assert (tgt.color[covered][:, :3] == [50, 100, 200]).all()       # R,G,B of texel (1,1)
+    # every covered pixel passes LEQUAL for both quads (pixels on a quad's
+    # diagonal are covered by both of its triangles: no fill rule)
+    assert st["fragments"] >= 2 * covered.sum()
+    pos2 = pos.copy()
+    pos2[:, 2] = -0.3                                                # second quad farther away
+    mesh2 = sg.Mesh(np.concatenate([pos, pos2]), mesh.nrm, mesh.uv, mesh.idx)
+    sc2 = sg.Scene(mesh2, 32, 32, sg.TEXTURE, False, (0.0, 0.0, 2.0), (1.0, 1.0, 1.0), tex, tex)
+    tgt2, _, _, _ = oracle.render_passes(sc2)
+    c2 = tgt2.depth < 1e30
+    assert (tgt2.color[c2 & covered][:, :3] == [30, 20, 10]).all()
+
+
+def test_uniforms_struct_size():
+    import ctypes
+    assert ctypes.sizeof(Uniforms) == 6 * 64 + 24 + 64 + 16
