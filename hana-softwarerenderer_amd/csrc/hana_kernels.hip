@@ -5,11 +5,14 @@
 //
 //   k_vertex   one vertex per lane: vertex program, clip-space trivial-accept
 //              bit, perspective divide and 1/w          (SR/IShader.cpp, SR/graphics.cpp:132-134, 316-337)
-//   k_setup<0> one face per lane: clipping of faces that fail the trivial
-//              accept (SR/graphics.cpp:136-161), back-face cull, viewport,
-//              bbox, degenerate test; counts (triangle, tile) pairs
+//   k_setup<0> one face per lane: back-face cull, viewport, bbox, degenerate
+//              test of trivially accepted faces; counts (triangle, tile)
+//              pairs; queues the other faces for clipping
+//   k_clip     clips the queued faces (SR/graphics.cpp:136-161) into polygons
+//              and counts their fan triangles
 //   k_scan     exclusive scan of the per-tile counts
 //   k_setup<1> same set-up; scatters triangle ids into the per-tile bins
+//   k_clip_scatter  scatters the clipped fan triangles
 //   k_raster   one workgroup per 32x32 tile: exact per-pixel coverage and
 //              depth (SR/graphics.cpp:222-233, 349-359) resolved in LDS with
 //              64-bit (depth, submission order) keys, then one fragment
@@ -252,18 +255,18 @@ __device__ __forceinline__ int clip_polygon(float (*a)[NF], float (*b)[NF]) {
 
 // Re-runs the vertex program on the face's corners, clips, and stores the
 // polygon (perspective-divided) in the pool.  Returns the polygon id or -1.
-__device__ __noinline__ int clip_face(const PassDesc& P, const Workspace& ws, int f, int i0, int i1, int i2) {
+__device__ __noinline__ int clip_face(const PassDesc& P, const Workspace& ws, int pass, int f) {
     const DevUniforms& u = *P.u;
     float a[POLY_MAX][4 + POLY_VARY], b[POLY_MAX][4 + POLY_VARY];
-    int ids[3] = {i0, i1, i2};
     bool need_n = P.kind != SHADER_SHADOW && P.kind != SHADER_TEXTURE;
     for (int k = 0; k < 3; k++) {
-        int v = ids[k];
+        int v = P.idx[3 * f + k];
         F3 p = f3(P.px[v], P.py[v], P.pz[v]);
         F3 n = need_n ? f3(P.nx[v], P.ny[v], P.nz[v]) : f3(0.f, 0.f, 0.f);
+        for (int c = 0; c < POLY_VARY; c++) a[k][4 + c] = 0.0f;
         vertex_program(P.kind, u, p, n, P.tu[v], P.tv[v], a[k], a[k] + 4);
     }
-    // positions only first: most such faces lie wholly outside and are dropped
+    // positions only first: many such faces lie wholly outside and are dropped
     {
         float pa[POLY_MAX][4], pb[POLY_MAX][4];
         for (int k = 0; k < 3; k++)
@@ -289,62 +292,150 @@ __device__ __noinline__ int clip_face(const PassDesc& P, const Workspace& ws, in
         for (int c = 0; c < POLY_VARY; c++) ws.pool_vs[((size_t)pid * POLY_MAX + k) * POLY_VARY + c] = b[k][4 + c];
     }
     ws.pool_face[pid] = f;
+    ws.pool_pass[pid] = pass;
     ws.pool_n[pid] = n;
     return pid;
 }
 
 // ---------------------------------------------------------------------------
-// k_setup: count (SCATTER = 0) or scatter (SCATTER = 1) (triangle, tile) pairs
+// binning: count (SCATTER = 0) or scatter (SCATTER = 1) (triangle, tile) pairs
 // ---------------------------------------------------------------------------
 
-template <int SCATTER>
-__device__ __forceinline__ void emit(const PassDesc& P, const Workspace& ws, const TriS& t, uint32_t entry) {
-    int tx0 = t.x0 >> TILE_SHIFT, tx1 = t.x1 >> TILE_SHIFT;
-    int ty0 = t.y0 >> TILE_SHIFT, ty1 = t.y1 >> TILE_SHIFT;
-    for (int ty = ty0; ty <= ty1; ty++)
-        for (int tx = tx0; tx <= tx1; tx++) {
-            int gt = P.tile0 + ty * P.tiles_x + tx;
-            if (!SCATTER) {
-                atomicAdd(&ws.tile_count[gt], 1u);
-            } else {
-                uint32_t pos = atomicAdd(&ws.tile_cursor[gt], 1u);
-                if (pos < ws.bin_cap) ws.bins[pos] = entry;
-            }
+// Reserves one slot in counters[t] for every lane with `want`, with one atomic
+// per distinct t in the wave; returns the lane's slot.  Every active lane of
+// the wave must call it together (the loop is wave-uniform).
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t* counters, int t, bool want) {
+    uint32_t mine = 0;
+    bool pending = want;
+    uint64_t mask = __ballot(pending);
+    int lane = __lane_id();
+    while (mask) {
+        int leader = __ffsll((unsigned long long)mask) - 1;
+        int lt = __shfl(t, leader);
+        bool match = pending && t == lt;
+        uint64_t same = __ballot(match);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&counters[lt], (uint32_t)__popcll(same));
+        base = __shfl(base, leader);
+        if (match) {
+            mine = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+            pending = false;
         }
-    if (!SCATTER) stat_add(P, ST_ENTRIES, (uint64_t)(tx1 - tx0 + 1) * (uint64_t)(ty1 - ty0 + 1));
+        mask = __ballot(pending);
+    }
+    return mine;
 }
 
+// Emits every (triangle, tile) pair of one lane's triangle; wave-uniform.
+template <int SCATTER>
+__device__ __forceinline__ void emit_tiles(const PassDesc& P, const Workspace& ws, bool ok, const TriS& t,
+                                           uint32_t entry) {
+    int tx0 = 0, ty0 = 0, nx = 0, n = 0;
+    if (ok) {
+        tx0 = t.x0 >> TILE_SHIFT;
+        ty0 = t.y0 >> TILE_SHIFT;
+        nx = (t.x1 >> TILE_SHIFT) - tx0 + 1;
+        n = nx * ((t.y1 >> TILE_SHIFT) - ty0 + 1);
+    }
+    for (int k = 0; __ballot(k < n); k++) {
+        bool want = k < n;
+        int gt = want ? P.tile0 + (ty0 + k / nx) * P.tiles_x + tx0 + k % nx : 0;
+        if (!SCATTER) {
+            wave_reserve(ws.tile_count, gt, want);
+        } else {
+            uint32_t pos = wave_reserve(ws.tile_cursor, gt, want);
+            if (want && pos < ws.bin_cap) ws.bins[pos] = entry;
+        }
+    }
+    if (!SCATTER && P.stats) {
+        uint64_t tot = 0;
+        for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off);
+        tot = (uint64_t)n;
+        if (__lane_id() == 0) stat_add(P, ST_ENTRIES, tot);
+    }
+}
+
+// One face per lane.  Trivially accepted faces are set up from the shared
+// post-transform vertices; the others are queued for k_clip.
 template <int SCATTER>
 __global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* passes, int npasses, Workspace ws) {
-    const PassDesc& P = passes[find_pass<1>(passes, npasses, blockIdx.x)];
+    int pass = find_pass<1>(passes, npasses, blockIdx.x);
+    const PassDesc& P = passes[pass];
     int f = (blockIdx.x - P.fblock0) * BLOCK + threadIdx.x;
-    if (f >= P.nfaces) return;
-    int i0 = P.idx[3 * f], i1 = P.idx[3 * f + 1], i2 = P.idx[3 * f + 2];
-    bool trivial = vis_bit(P.vis, i0) && vis_bit(P.vis, i1) && vis_bit(P.vis, i2);
+    bool valid = f < P.nfaces;
+    bool trivial = false;
+    int i0 = 0, i1 = 0, i2 = 0;
+    if (valid) {
+        i0 = P.idx[3 * f];
+        i1 = P.idx[3 * f + 1];
+        i2 = P.idx[3 * f + 2];
+        trivial = vis_bit(P.vis, i0) && vis_bit(P.vis, i1) && vis_bit(P.vis, i2);
+    }
     if (!SCATTER) {
-        stat_add(P, ST_FACES, 1);
-        if (!trivial) stat_add(P, ST_CLIPPED, 1);
+        bool clipq = valid && !trivial;
+        uint32_t q = wave_reserve(&ws.status[STATUS_CLIPQ], 0, clipq);
+        if (clipq && q < ws.clipq_cap) ws.clip_queue[q] = ((uint64_t)(uint32_t)pass << 32) | (uint32_t)f;
+        if (P.stats) {
+            uint64_t nv = __popcll(__ballot(valid)), nc = __popcll(__ballot(clipq));
+            if (__lane_id() == 0) {
+                stat_add(P, ST_FACES, nv);
+                stat_add(P, ST_CLIPPED, nc);
+            }
+        }
     }
-    if (trivial) {
-        TriS t;
-        if (tri_setup(P.vr[i0], P.vr[i1], P.vr[i2], P.width, P.height, t)) emit<SCATTER>(P, ws, t, (uint32_t)f);
-        return;
+    TriS t;
+    bool ok = trivial && tri_setup(P.vr[i0], P.vr[i1], P.vr[i2], P.width, P.height, t);
+    emit_tiles<SCATTER>(P, ws, ok, t, (uint32_t)f);
+}
+
+// Clips the queued faces (count phase): grid-stride over the queue.
+__global__ __launch_bounds__(BLOCK) void k_clip(const PassDesc* passes, Workspace ws) {
+    uint32_t count = min(ws.status[STATUS_CLIPQ], ws.clipq_cap);
+    if (ws.status[STATUS_CLIPQ] > ws.clipq_cap && blockIdx.x == 0 && threadIdx.x == 0)
+        atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_CLIPQ_OVERFLOW);
+    for (uint32_t base = blockIdx.x * BLOCK; base < count; base += gridDim.x * BLOCK) {
+        uint32_t q = base + threadIdx.x;
+        bool have = q < count;
+        int pass = 0, f = 0, pid = -1;
+        if (have) {
+            uint64_t e = ws.clip_queue[q];
+            pass = (int)(e >> 32);
+            f = (int)(uint32_t)e;
+            pid = clip_face(passes[pass], ws, pass, f);
+            passes[pass].poly_of_face[f] = pid;
+            if (pid >= 0) stat_add(passes[pass], ST_POLYS, 1);
+        }
+        int n = pid >= 0 ? ws.pool_n[pid] : 0;
+        // fan triangles (0, j+1, j+2), counted per j across the wave
+        for (int j = 0; __ballot(j < n - 2); j++) {
+            TriS t;
+            bool ok = false;
+            if (j < n - 2) {
+                const float4* pv = ws.pool_vr + (size_t)pid * POLY_MAX;
+                ok = tri_setup(pv[0], pv[j + 1], pv[j + 2], passes[pass].width, passes[pass].height, t);
+            }
+            emit_tiles<0>(passes[pass], ws, ok, t, 0u);
+        }
     }
-    int pid;
-    if (!SCATTER) {
-        pid = clip_face(P, ws, f, i0, i1, i2);
-        P.poly_of_face[f] = pid;
-        if (pid >= 0) stat_add(P, ST_POLYS, 1);
-    } else {
-        pid = P.poly_of_face[f];
-    }
-    if (pid < 0) return;
-    int n = ws.pool_n[pid];
-    const float4* pv = ws.pool_vr + (size_t)pid * POLY_MAX;
-    for (int j = 0; j < n - 2; j++) {
-        TriS t;
-        if (tri_setup(pv[0], pv[j + 1], pv[j + 2], P.width, P.height, t))
-            emit<SCATTER>(P, ws, t, CLIPPED_BIT | ((uint32_t)pid << 3) | (uint32_t)j);
+}
+
+// Scatters the fan triangles of every clipped polygon: grid-stride over the pool.
+__global__ __launch_bounds__(BLOCK) void k_clip_scatter(const PassDesc* passes, Workspace ws) {
+    uint32_t count = min(ws.status[STATUS_POLYS], ws.poly_cap);
+    for (uint32_t base = blockIdx.x * BLOCK; base < count; base += gridDim.x * BLOCK) {
+        uint32_t pid = base + threadIdx.x;
+        bool have = pid < count;
+        int pass = have ? ws.pool_pass[pid] : 0;
+        int n = have ? ws.pool_n[pid] : 0;
+        for (int j = 0; __ballot(j < n - 2); j++) {
+            TriS t;
+            bool ok = false;
+            if (j < n - 2) {
+                const float4* pv = ws.pool_vr + (size_t)pid * POLY_MAX;
+                ok = tri_setup(pv[0], pv[j + 1], pv[j + 2], passes[pass].width, passes[pass].height, t);
+            }
+            emit_tiles<1>(passes[pass], ws, ok, t, CLIPPED_BIT | (pid << 3) | (uint32_t)j);
+        }
     }
 }
 
@@ -618,6 +709,13 @@ hipError_t launch_setup(const PassDesc* d_passes, int npasses, const Workspace& 
     if (blocks <= 0) return hipSuccess;
     if (scatter) hipLaunchKernelGGL(k_setup<1>, dim3(blocks), dim3(BLOCK), 0, s, d_passes, npasses, ws);
     else hipLaunchKernelGGL(k_setup<0>, dim3(blocks), dim3(BLOCK), 0, s, d_passes, npasses, ws);
+    return hipGetLastError();
+}
+hipError_t launch_clip(const PassDesc* d_passes, const Workspace& ws, int scatter, hipStream_t s) {
+    // grid-stride kernels: the queue length is only known on the device
+    const int blocks = 512;
+    if (scatter) hipLaunchKernelGGL(k_clip_scatter, dim3(blocks), dim3(BLOCK), 0, s, d_passes, ws);
+    else hipLaunchKernelGGL(k_clip, dim3(blocks), dim3(BLOCK), 0, s, d_passes, ws);
     return hipGetLastError();
 }
 hipError_t launch_scan(const Workspace& ws, hipStream_t s) {

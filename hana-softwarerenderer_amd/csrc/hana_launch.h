@@ -9,6 +9,7 @@ namespace hana {
 hipError_t launch_vertex(const PassDesc* d_passes, int npasses, const Workspace& ws, int blocks, hipStream_t s);
 hipError_t launch_setup(const PassDesc* d_passes, int npasses, const Workspace& ws, int blocks, int scatter,
                         hipStream_t s);
+hipError_t launch_clip(const PassDesc* d_passes, const Workspace& ws, int scatter, hipStream_t s);
 hipError_t launch_scan(const Workspace& ws, hipStream_t s);
 hipError_t launch_raster(const PassDesc* d_passes, int npasses, const Workspace& ws, int blocks, hipStream_t s);
 hipError_t launch_fill32(uint32_t* dst, uint32_t value, size_t n, hipStream_t s);

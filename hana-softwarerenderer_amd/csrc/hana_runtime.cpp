@@ -66,8 +66,8 @@ struct PassSpec {
 // Grow-only device workspace for launch sequences.
 struct Arena {
     DevBuf descs, unis, vr, vs, vis, pof, tile_count, tile_offset, tile_cursor, bins;
-    DevBuf pool_vr, pool_vs, pool_face, pool_n, status, stats;
-    uint32_t bin_cap = 0, poly_cap = 0;
+    DevBuf pool_vr, pool_vs, pool_face, pool_pass, pool_n, clip_queue, status, stats;
+    uint32_t bin_cap = 0, poly_cap = 0, clipq_cap = 0;
 };
 
 struct Plan {
@@ -103,6 +103,10 @@ struct hana_ctx {
     uint64_t* stats_host = nullptr;    // pinned, per pass
     int stats_passes = 0;
     bool status_pending = false;
+    // optional per-stage timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
 };
 
 struct hana_batch {
@@ -268,8 +272,16 @@ int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_p
         if ((r = ensure(c, A.pool_vr, sizeof(float4) * POLY_MAX * want_polys))) return r;
         if ((r = ensure(c, A.pool_vs, sizeof(float) * POLY_MAX * POLY_VARY * want_polys))) return r;
         if ((r = ensure(c, A.pool_face, sizeof(int32_t) * want_polys))) return r;
+        if ((r = ensure(c, A.pool_pass, sizeof(int32_t) * want_polys))) return r;
         if ((r = ensure(c, A.pool_n, sizeof(int32_t) * want_polys))) return r;
         A.poly_cap = (uint32_t)want_polys;
+    }
+    // every face may fail the trivial accept (camera inside the mesh)
+    uint64_t want_q = std::max<uint64_t>(pl.total_f, 1024);
+    want_q = std::min<uint64_t>(want_q, 0xFFFFFFF0ull);
+    if (want_q > A.clipq_cap) {
+        if ((r = ensure(c, A.clip_queue, sizeof(uint64_t) * want_q))) return r;
+        A.clipq_cap = (uint32_t)want_q;
     }
     if ((r = ensure(c, A.status, sizeof(uint32_t) * STATUS_WORDS))) return r;
     if ((r = ensure(c, A.stats, sizeof(uint64_t) * ST_WORDS * std::max(n, 1)))) return r;
@@ -293,7 +305,10 @@ int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_p
     ws.pool_vr = A.pool_vr.as<float4>();
     ws.pool_vs = A.pool_vs.as<float>();
     ws.pool_face = A.pool_face.as<int32_t>();
+    ws.pool_pass = A.pool_pass.as<int32_t>();
     ws.pool_n = A.pool_n.as<int32_t>();
+    ws.clip_queue = A.clip_queue.as<uint64_t>();
+    ws.clipq_cap = A.clipq_cap;
     ws.status = A.status.as<uint32_t>();
     ws.ntiles = pl.total_tiles;
     ws.bin_cap = A.bin_cap;
@@ -316,16 +331,75 @@ void take_clears(Plan& pl) {
     }
 }
 
-int enqueue_sequence(hana_ctx* c, const Plan& pl, const Arena& A, hipStream_t s) {
+hipEvent_t pool_event(hana_ctx* c) {
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Records an event pair around one launch when timing is on (never inside a
+// graph capture: `timed` is false there).
+struct StageTimer {
+    hana_ctx* c;
+    int stage;
+    bool on;
+    hipEvent_t a = nullptr, b = nullptr;
+    StageTimer(hana_ctx* c_, int st, bool timed, hipStream_t s) : c(c_), stage(st), on(timed && c_->timing) {
+        if (on) {
+            a = pool_event(c);
+            b = pool_event(c);
+            on = a && b && hipEventRecord(a, s) == hipSuccess;
+        }
+    }
+    void done(hipStream_t s) {
+        if (on && hipEventRecord(b, s) == hipSuccess) c->ev_pending.push_back({stage, {a, b}});
+    }
+};
+
+int enqueue_sequence(hana_ctx* c, const Plan& pl, const Arena& A, hipStream_t s, bool timed = true) {
     const PassDesc* d = A.descs.as<PassDesc>();
     int n = (int)pl.descs.size();
-    HIP_TRY(c, launch_vertex(d, n, pl.ws, pl.vblocks, s));
-    HIP_TRY(c, launch_setup(d, n, pl.ws, pl.fblocks, 0, s));
-    HIP_TRY(c, launch_scan(pl.ws, s));
-    HIP_TRY(c, launch_setup(d, n, pl.ws, pl.fblocks, 1, s));
+    {
+        StageTimer t(c, 0, timed, s);
+        HIP_TRY(c, launch_vertex(d, n, pl.ws, pl.vblocks, s));
+        t.done(s);
+    }
+    {
+        StageTimer t(c, 1, timed, s);
+        HIP_TRY(c, launch_setup(d, n, pl.ws, pl.fblocks, 0, s));
+        t.done(s);
+    }
+    {
+        StageTimer t(c, 2, timed, s);
+        HIP_TRY(c, launch_clip(d, pl.ws, 0, s));
+        t.done(s);
+    }
+    {
+        StageTimer t(c, 3, timed, s);
+        HIP_TRY(c, launch_scan(pl.ws, s));
+        t.done(s);
+    }
+    {
+        StageTimer t(c, 4, timed, s);
+        HIP_TRY(c, launch_setup(d, n, pl.ws, pl.fblocks, 1, s));
+        t.done(s);
+    }
+    {
+        StageTimer t(c, 5, timed, s);
+        HIP_TRY(c, launch_clip(d, pl.ws, 1, s));
+        t.done(s);
+    }
     for (int g = 0; g < 2; g++)
-        if (pl.group_count[g])
+        if (pl.group_count[g]) {
+            StageTimer t(c, 6 + g, timed, s);
             HIP_TRY(c, launch_raster(d + pl.group_first[g], pl.group_count[g], pl.ws, pl.group_blocks[g], s));
+            t.done(s);
+        }
     return HANA_OK;
 }
 
@@ -378,6 +452,7 @@ int check_status(hana_ctx* c) {
             if ((r = ensure(c, c->arena.pool_vr, sizeof(float4) * POLY_MAX * want))) return r;
             if ((r = ensure(c, c->arena.pool_vs, sizeof(float) * POLY_MAX * POLY_VARY * want))) return r;
             if ((r = ensure(c, c->arena.pool_face, sizeof(int32_t) * want))) return r;
+            if ((r = ensure(c, c->arena.pool_pass, sizeof(int32_t) * want))) return r;
             if ((r = ensure(c, c->arena.pool_n, sizeof(int32_t) * want))) return r;
             c->arena.poly_cap = (uint32_t)want;
         }
@@ -465,7 +540,8 @@ int hana_gpu_create(int device, hana_ctx** out) {
 
 static void free_arena(Arena& a) {
     DevBuf* bufs[] = {&a.descs, &a.unis, &a.vr, &a.vs, &a.vis, &a.pof, &a.tile_count, &a.tile_offset,
-                      &a.tile_cursor, &a.bins, &a.pool_vr, &a.pool_vs, &a.pool_face, &a.pool_n, &a.status, &a.stats};
+                      &a.tile_cursor, &a.bins, &a.pool_vr, &a.pool_vs, &a.pool_face, &a.pool_pass, &a.pool_n,
+                      &a.clip_queue, &a.status, &a.stats};
     for (DevBuf* b : bufs)
         if (b->p) {
             (void)hipFree(b->p);
@@ -485,6 +561,11 @@ int hana_gpu_destroy(hana_ctx* c) {
     }
     if (c->status_host) (void)hipHostFree(c->status_host);
     if (c->stats_host) (void)hipHostFree(c->stats_host);
+    for (auto& p : c->ev_pending) {
+        (void)hipEventDestroy(p.second.first);
+        (void)hipEventDestroy(p.second.second);
+    }
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return HANA_OK;
@@ -496,6 +577,31 @@ int hana_gpu_sync(hana_ctx* c) {
     if (!c) return HANA_E_INVALID;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return check_status(c);
+}
+
+int hana_gpu_set_timing(hana_ctx* c, int enable) {
+    if (!c) return HANA_E_INVALID;
+    c->timing = enable != 0;
+    return HANA_OK;
+}
+
+int hana_gpu_read_timing(hana_ctx* c, double* ms, int64_t* launches) {
+    if (!c || !ms || !launches) return HANA_E_INVALID;
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < HANA_TIMING_STAGES; i++) {
+        ms[i] = 0.0;
+        launches[i] = 0;
+    }
+    for (auto& p : c->ev_pending) {
+        float t = 0.f;
+        HIP_TRY(c, hipEventElapsedTime(&t, p.second.first, p.second.second));
+        ms[p.first] += t;
+        launches[p.first] += 1;
+        c->ev_pool.push_back(p.second.first);
+        c->ev_pool.push_back(p.second.second);
+    }
+    c->ev_pending.clear();
+    return HANA_OK;
 }
 
 int hana_gpu_set_stats(hana_ctx* c, int enable) {
@@ -772,7 +878,7 @@ int hana_batch_finalize(hana_ctx* c, hana_batch* b, int use_graph) {
     b->use_graph = use_graph != 0;
     if (b->use_graph) {
         HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int er = enqueue_sequence(c, b->plan, b->arena, c->stream);
+        int er = enqueue_sequence(c, b->plan, b->arena, c->stream, false);
         hipGraph_t g = nullptr;
         hipError_t ec = hipStreamEndCapture(c->stream, &g);
         if (er) return er;

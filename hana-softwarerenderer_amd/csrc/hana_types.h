@@ -85,16 +85,19 @@ struct Workspace {
     float4* pool_vr;             // clipped polygons: POLY_MAX vertices each
     float* pool_vs;              //   ... POLY_VARY varyings per vertex
     int32_t* pool_face;          //   face (within its pass) of each polygon
+    int32_t* pool_pass;          //   pass of each polygon
     int32_t* pool_n;             //   vertex count of each polygon
-    uint32_t* status;            // [0] poly count, [1] overflow flags, [2] bins needed, [3] polys needed
+    uint64_t* clip_queue;        // faces failing the trivial accept: pass << 32 | face
+    uint32_t* status;            // see STATUS_*
     int32_t ntiles;
     uint32_t bin_cap;
     uint32_t poly_cap;
-    int32_t pad_;
+    uint32_t clipq_cap;
 };
 
-enum { STATUS_POLYS = 0, STATUS_FLAGS = 1, STATUS_BINS_NEEDED = 2, STATUS_POLYS_NEEDED = 3, STATUS_WORDS = 4 };
-enum { FLAG_BIN_OVERFLOW = 1, FLAG_POLY_OVERFLOW = 2, FLAG_POLY_TOO_LARGE = 4 };
+// status words: polygons allocated, overflow flags, bin entries needed, faces queued for clipping
+enum { STATUS_POLYS = 0, STATUS_FLAGS = 1, STATUS_BINS_NEEDED = 2, STATUS_CLIPQ = 3, STATUS_WORDS = 4 };
+enum { FLAG_BIN_OVERFLOW = 1, FLAG_POLY_OVERFLOW = 2, FLAG_POLY_TOO_LARGE = 4, FLAG_CLIPQ_OVERFLOW = 8 };
 enum { ST_FACES = 0, ST_CLIPPED = 1, ST_POLYS = 2, ST_ENTRIES = 3, ST_PIXELS = 4, ST_WORDS = 5 };
 
 }  // namespace hana

@@ -58,6 +58,8 @@ DECLS = [
     ("hana_gpu_sync", C.c_int, [_P]),
     ("hana_gpu_set_stats", C.c_int, [_P, C.c_int]),
     ("hana_gpu_last_stats", C.c_int, [_P, C.POINTER(PassStats), C.c_int, C.POINTER(C.c_int)]),
+    ("hana_gpu_set_timing", C.c_int, [_P, C.c_int]),
+    ("hana_gpu_read_timing", C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     ("hana_mesh_create", C.c_int, [_P, C.c_int32, _FP, _FP, _FP, C.c_int32, C.POINTER(C.c_int32), C.POINTER(_P)]),
     ("hana_mesh_destroy", C.c_int, [_P, _P]),
     ("hana_texture_create", C.c_int, [_P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_uint8), C.POINTER(_P)]),
@@ -173,6 +175,18 @@ class Context:
         n = C.c_int()
         self.check(self.lib.hana_gpu_last_stats(self.h, arr, max_passes, C.byref(n)))
         return [arr[i].as_dict() for i in range(n.value)]
+
+    STAGES = ("vertex", "setup_count", "clip_count", "scan", "setup_scatter", "clip_scatter", "raster_g0",
+              "raster_g1")
+
+    def set_timing(self, on: bool):
+        self.check(self.lib.hana_gpu_set_timing(self.h, int(on)))
+
+    def read_timing(self):
+        ms = (C.c_double * len(self.STAGES))()
+        n = (C.c_int64 * len(self.STAGES))()
+        self.check(self.lib.hana_gpu_read_timing(self.h, ms, n))
+        return {s: (float(ms[i]), int(n[i])) for i, s in enumerate(self.STAGES)}
 
     # -- resources --
     def mesh(self, m):

@@ -117,6 +117,16 @@ int hana_gpu_sync(hana_ctx* ctx);
 int hana_gpu_set_stats(hana_ctx* ctx, int enable);
 int hana_gpu_last_stats(hana_ctx* ctx, hana_pass_stats* out, int max_passes, int* n_passes);
 
+/* Per-stage kernel timing with HIP events recorded on the context stream
+ * around each launch (benchmarking).  Stages: 0 vertex, 1 set-up/count,
+ * 2 clip/count, 3 scan, 4 set-up/scatter, 5 clip/scatter, 6 raster of the
+ * first dependency group (shadow passes), 7 raster of the second group (main
+ * passes).  Reading returns the milliseconds and launch counts accumulated
+ * since the last read. */
+#define HANA_TIMING_STAGES 8
+int hana_gpu_set_timing(hana_ctx* ctx, int enable);
+int hana_gpu_read_timing(hana_ctx* ctx, double* ms, int64_t* launches);
+
 /* ---- resources ---- */
 /* SoA fp32 attributes (pos 3V, nrm 3V, uv 2V) and int32 indices (3F); one
  * index per corner for position, normal and uv.  Normals are used as
