@@ -130,14 +130,20 @@ def _smooth(t):
 
 
 def value_noise_2d(x: np.ndarray, y: np.ndarray, seed: int) -> np.ndarray:
+    """Value noise on the grid x (columns, 1-D) by y (rows, 1-D): (len(y), len(x))."""
     xi = np.floor(x).astype(np.int64)
     yi = np.floor(y).astype(np.int64)
-    tx = _smooth(x - xi)
-    ty = _smooth(y - yi)
-    def g(dx, dy):
-        return _hash_u32(xi + dx, yi + dy, seed=seed).astype(np.float64) / 4294967296.0
-    a = g(0, 0) + (g(1, 0) - g(0, 0)) * tx
-    b = g(0, 1) + (g(1, 1) - g(0, 1)) * tx
+    tx = _smooth(x - xi)[None, :]
+    ty = _smooth(y - yi)[:, None]
+    # hash the lattice points once, then gather rows and columns
+    x0, y0 = int(xi.min()), int(yi.min())
+    lx = np.arange(x0, int(xi.max()) + 2, dtype=np.int64)
+    ly = np.arange(y0, int(yi.max()) + 2, dtype=np.int64)
+    lat = _hash_u32(lx[None, :], ly[:, None], seed=seed).astype(np.float64) / 4294967296.0
+    ix, iy = xi - x0, yi - y0
+    r0, r1 = lat[iy], lat[iy + 1]
+    a = r0[:, ix] + (r0[:, ix + 1] - r0[:, ix]) * tx
+    b = r1[:, ix] + (r1[:, ix + 1] - r1[:, ix]) * tx
     return a + (b - a) * ty
 
 
@@ -158,7 +164,7 @@ def value_noise_3d(p: np.ndarray, seed: int) -> np.ndarray:
 def fbm_2d(w: int, h: int, seed: int, base_cells: float = 8.0, octaves: int = 4,
            persistence: float = 0.5) -> np.ndarray:
     """4-octave value noise over a w x h image, in [0, 1)."""
-    ys, xs = np.meshgrid(np.arange(h, dtype=np.float64), np.arange(w, dtype=np.float64), indexing="ij")
+    ys, xs = np.arange(h, dtype=np.float64), np.arange(w, dtype=np.float64)
     acc = np.zeros((h, w))
     amp, tot = 1.0, 0.0
     for o in range(octaves):
@@ -306,10 +312,22 @@ def icosphere(levels: int) -> tuple[np.ndarray, np.ndarray]:
         v = np.concatenate([v, mid], axis=0)
         f0, f1, f2 = f[:, 0], f[:, 1], f[:, 2]
         m01, m12, m20 = m[:, 0], m[:, 1], m[:, 2]
-        f = np.concatenate([
+        # the four children of face i are faces 4i..4i+3: submission order
+        # stays spatially coherent at every level
+        f = np.stack([
             np.stack([f0, m01, m20], 1), np.stack([f1, m12, m01], 1),
-            np.stack([f2, m20, m12], 1), np.stack([m01, m12, m20], 1)], axis=0)
-    return v, f
+            np.stack([f2, m20, m12], 1), np.stack([m01, m12, m20], 1)], axis=1).reshape(-1, 3)
+    return reorder_vertices(v, f)
+
+
+def reorder_vertices(v: np.ndarray, f: np.ndarray):
+    """Renumbers vertices in order of first use by the faces."""
+    flat = f.ravel()
+    uniq, first = np.unique(flat, return_index=True)
+    order = uniq[np.argsort(first)]
+    remap = np.empty(v.shape[0], np.int64)
+    remap[order] = np.arange(order.size)
+    return v[order], remap[f]
 
 
 def spherical_uv(unit: np.ndarray) -> np.ndarray:
@@ -359,7 +377,7 @@ def box(n: int) -> Mesh:
             faces_pos.append(p)
             faces_nrm.append(nn)
             faces_uv.append(uvs)
-            faces_idx.append(np.concatenate([t1, t2]) + base)
+            faces_idx.append(np.stack([t1, t2], 1).reshape(-1, 3) + base)
             base += a.size
     pos = np.concatenate(faces_pos).astype(np.float32)
     idx = _orient_outward(pos, np.concatenate(faces_idx).astype(np.int32))
@@ -535,7 +553,7 @@ def config5_scene(layers: int = 8, grid: int = 1119, width: int = 3840, height: 
     q = np.arange(grid * grid).reshape(grid, grid)
     t1 = np.stack([q[:-1, :-1].ravel(), q[:-1, 1:].ravel(), q[1:, :-1].ravel()], 1)
     t2 = np.stack([q[:-1, 1:].ravel(), q[1:, 1:].ravel(), q[1:, :-1].ravel()], 1)
-    idx = np.concatenate([t1, t2]).astype(np.int32)
+    idx = np.stack([t1, t2], 1).reshape(-1, 3).astype(np.int32)
     for pair in range(len(depths)):
         z = float(depths[pair])
         dist = cam_z - z
