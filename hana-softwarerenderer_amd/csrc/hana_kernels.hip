@@ -214,25 +214,42 @@ __device__ __forceinline__ float intersect_ratio(const float* p, const float* c,
     }
 }
 
-// One Sutherland-Hodgman stage over vertices of NF floats (clip pos first);
-// every float is interpolated as liner_interpolate_varyings does.  A convex
-// polygon gains at most one vertex per plane (3 + 7 = POLY_MAX); an output
-// that would exceed POLY_MAX (only possible through rounding) returns -1 and
-// the polygon is dropped and flagged.
+// Polygon buffers of one lane, kept in LDS: element (vertex v, float k) of
+// the lane at base[(v * NF + k) * CLIP_LANES + lane], so that the lanes of a
+// wave touch consecutive words (no bank conflicts, no scratch memory).
+constexpr int CLIP_LANES = 64;
+constexpr int CLIP_NF = 4 + POLY_VARY;
 template <int NF>
-__device__ __forceinline__ int clip_stage(int plane, int n, const float (*in)[NF], float (*out)[NF]) {
+struct LanePoly {
+    float* base;
+    int lane;
+    __device__ __forceinline__ float& at(int v, int k) const { return base[(v * NF + k) * CLIP_LANES + lane]; }
+};
+
+// One Sutherland-Hodgman stage over vertices of NF floats (clip pos first);
+// every float is interpolated as liner_interpolate_varyings does
+// (SR/graphics.cpp:7-17, 73-110).  A convex polygon gains at most one vertex
+// per plane (3 + 7 = POLY_MAX); an output that would exceed POLY_MAX (only
+// possible through rounding) returns -1 and the polygon is dropped and flagged.
+template <int NF>
+__device__ __forceinline__ int clip_stage(int plane, int n, LanePoly<NF> in, LanePoly<NF> out) {
     int m = 0;
     for (int i = 0; i < n; i++) {
         int prev = (i - 1 + n) % n;
-        bool pin = inside_plane(in[prev], plane), cin = inside_plane(in[i], plane);
+        float pc[4], cc[4];
+        for (int k = 0; k < 4; k++) { pc[k] = in.at(prev, k); cc[k] = in.at(i, k); }
+        bool pin = inside_plane(pc, plane), cin = inside_plane(cc, plane);
         if (m + (pin != cin) + (cin ? 1 : 0) > POLY_MAX) return -1;
         if (pin != cin) {
-            float t = intersect_ratio(in[prev], in[i], plane);
-            for (int k = 0; k < NF; k++) out[m][k] = in[prev][k] + (in[i][k] - in[prev][k]) * t;
+            float t = intersect_ratio(pc, cc, plane);
+            for (int k = 0; k < NF; k++) {
+                float f = in.at(prev, k);
+                out.at(m, k) = f + (in.at(i, k) - f) * t;
+            }
             m++;
         }
         if (cin) {
-            for (int k = 0; k < NF; k++) out[m][k] = in[i][k];
+            for (int k = 0; k < NF; k++) out.at(m, k) = in.at(i, k);
             m++;
         }
     }
@@ -241,7 +258,7 @@ __device__ __forceinline__ int clip_stage(int plane, int n, const float (*in)[NF
 // Returns the vertex count (result in b), 0 when fewer than 3 remain, -1 on
 // a polygon beyond POLY_MAX vertices.
 template <int NF>
-__device__ __forceinline__ int clip_polygon(float (*a)[NF], float (*b)[NF]) {
+__device__ __forceinline__ int clip_polygon(LanePoly<NF> a, LanePoly<NF> b) {
     int n = 3;
     n = clip_stage<NF>(PL_W, n, a, b); if (n < 3) return n < 0 ? -1 : 0;
     n = clip_stage<NF>(PL_PX, n, b, a); if (n < 3) return n < 0 ? -1 : 0;
@@ -255,27 +272,34 @@ __device__ __forceinline__ int clip_polygon(float (*a)[NF], float (*b)[NF]) {
 
 // Re-runs the vertex program on the face's corners, clips, and stores the
 // polygon (perspective-divided) in the pool.  Returns the polygon id or -1.
-__device__ __noinline__ int clip_face(const PassDesc& P, const Workspace& ws, int pass, int f) {
+// lds: 2 * POLY_MAX * CLIP_NF * CLIP_LANES floats shared by the wave.
+__device__ __noinline__ int clip_face(const PassDesc& P, const Workspace& ws, int pass, int f, float* lds, int lane) {
     const DevUniforms& u = *P.u;
-    float a[POLY_MAX][4 + POLY_VARY], b[POLY_MAX][4 + POLY_VARY];
+    LanePoly<CLIP_NF> a{lds, lane};
+    LanePoly<CLIP_NF> b{lds + POLY_MAX * CLIP_NF * CLIP_LANES, lane};
     bool need_n = P.kind != SHADER_SHADOW && P.kind != SHADER_TEXTURE;
     for (int k = 0; k < 3; k++) {
         int v = P.idx[3 * f + k];
         F3 p = f3(P.px[v], P.py[v], P.pz[v]);
         F3 n = need_n ? f3(P.nx[v], P.ny[v], P.nz[v]) : f3(0.f, 0.f, 0.f);
-        for (int c = 0; c < POLY_VARY; c++) a[k][4 + c] = 0.0f;
-        vertex_program(P.kind, u, p, n, P.tu[v], P.tv[v], a[k], a[k] + 4);
+        float clip[4], vary[8];
+        for (int c = 0; c < 8; c++) vary[c] = 0.0f;
+        vertex_program(P.kind, u, p, n, P.tu[v], P.tv[v], clip, vary);
+        for (int c = 0; c < 4; c++) a.at(k, c) = clip[c];
+        for (int c = 0; c < POLY_VARY; c++) a.at(k, 4 + c) = vary[c];
     }
-    // positions only first: many such faces lie wholly outside and are dropped
+    // positions only first (in b's space): many such faces lie wholly
+    // outside and are dropped without interpolating varyings
     {
-        float pa[POLY_MAX][4], pb[POLY_MAX][4];
+        LanePoly<4> pa{b.base, lane};
+        LanePoly<4> pb{b.base + POLY_MAX * 4 * CLIP_LANES, lane};
         for (int k = 0; k < 3; k++)
-            for (int c = 0; c < 4; c++) pa[k][c] = a[k][c];
+            for (int c = 0; c < 4; c++) pa.at(k, c) = a.at(k, c);
         int np = clip_polygon<4>(pa, pb);
         if (np < 0) atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_POLY_TOO_LARGE);
         if (np < 3) return -1;
     }
-    int n = clip_polygon<4 + POLY_VARY>(a, b);
+    int n = clip_polygon<CLIP_NF>(a, b);
     if (n < 3) return -1;
     int pid = (int)atomicAdd(&ws.status[STATUS_POLYS], 1u);
     if ((uint32_t)pid >= ws.poly_cap) {
@@ -283,13 +307,14 @@ __device__ __noinline__ int clip_face(const PassDesc& P, const Workspace& ws, in
         return -1;
     }
     for (int k = 0; k < n; k++) {
+        float w = b.at(k, 3);
         float4 r;
-        r.w = 1 / b[k][3];
-        r.z = b[k][2] / b[k][3];
-        r.y = b[k][1] / b[k][3];
-        r.x = b[k][0] / b[k][3];
+        r.w = 1 / w;
+        r.z = b.at(k, 2) / w;
+        r.y = b.at(k, 1) / w;
+        r.x = b.at(k, 0) / w;
         ws.pool_vr[(size_t)pid * POLY_MAX + k] = r;
-        for (int c = 0; c < POLY_VARY; c++) ws.pool_vs[((size_t)pid * POLY_MAX + k) * POLY_VARY + c] = b[k][4 + c];
+        for (int c = 0; c < POLY_VARY; c++) ws.pool_vs[((size_t)pid * POLY_MAX + k) * POLY_VARY + c] = b.at(k, 4 + c);
     }
     ws.pool_face[pid] = f;
     ws.pool_pass[pid] = pass;
@@ -388,12 +413,14 @@ __global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* passes, int npa
     emit_tiles<SCATTER>(P, ws, ok, t, (uint32_t)f);
 }
 
-// Clips the queued faces (count phase): grid-stride over the queue.
-__global__ __launch_bounds__(BLOCK) void k_clip(const PassDesc* passes, Workspace ws) {
+// Clips the queued faces (count phase): one wave per workgroup, grid-stride
+// over the queue, polygon buffers in LDS.
+__global__ __launch_bounds__(CLIP_LANES) void k_clip(const PassDesc* passes, Workspace ws) {
+    __shared__ float lds[2 * POLY_MAX * CLIP_NF * CLIP_LANES];
     uint32_t count = min(ws.status[STATUS_CLIPQ], ws.clipq_cap);
     if (ws.status[STATUS_CLIPQ] > ws.clipq_cap && blockIdx.x == 0 && threadIdx.x == 0)
         atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_CLIPQ_OVERFLOW);
-    for (uint32_t base = blockIdx.x * BLOCK; base < count; base += gridDim.x * BLOCK) {
+    for (uint32_t base = blockIdx.x * CLIP_LANES; base < count; base += gridDim.x * CLIP_LANES) {
         uint32_t q = base + threadIdx.x;
         bool have = q < count;
         int pass = 0, f = 0, pid = -1;
@@ -401,7 +428,7 @@ __global__ __launch_bounds__(BLOCK) void k_clip(const PassDesc* passes, Workspac
             uint64_t e = ws.clip_queue[q];
             pass = (int)(e >> 32);
             f = (int)(uint32_t)e;
-            pid = clip_face(passes[pass], ws, pass, f);
+            pid = clip_face(passes[pass], ws, pass, f, lds, threadIdx.x);
             passes[pass].poly_of_face[f] = pid;
             if (pid >= 0) stat_add(passes[pass], ST_POLYS, 1);
         }
@@ -632,10 +659,22 @@ __device__ __forceinline__ void shade_pixel(const PassDesc& P, const Workspace& 
     if (P.winner) P.winner[pix] = order;
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin by blockIdx; the
+// logical tile of a workgroup is chosen so that each XCD receives one
+// contiguous run of tiles (a band of the target), whose vertices, bins and
+// texels then stay in that XCD's L2.
+__device__ __forceinline__ int xcd_logical_block(int b, int nb) {
+    const int kXcd = 8;
+    int xcd = b % kXcd, slot = b / kXcd;
+    int per = nb / kXcd, rem = nb % kXcd;
+    return xcd * per + min(xcd, rem) + slot;
+}
+
 __global__ __launch_bounds__(BLOCK) void k_raster(const PassDesc* passes, int npasses, Workspace ws) {
     __shared__ TileLds L;
-    const PassDesc& P = passes[find_pass<2>(passes, npasses, blockIdx.x)];
-    int lt = blockIdx.x - P.tblock0;
+    int blk = xcd_logical_block(blockIdx.x, gridDim.x);
+    const PassDesc& P = passes[find_pass<2>(passes, npasses, blk)];
+    int lt = blk - P.tblock0;
     int tx = lt % P.tiles_x, ty = lt / P.tiles_x;
     int gt = P.tile0 + lt;
     int xb = tx << TILE_SHIFT, yb = ty << TILE_SHIFT;
@@ -715,7 +754,7 @@ hipError_t launch_clip(const PassDesc* d_passes, const Workspace& ws, int scatte
     // grid-stride kernels: the queue length is only known on the device
     const int blocks = 512;
     if (scatter) hipLaunchKernelGGL(k_clip_scatter, dim3(blocks), dim3(BLOCK), 0, s, d_passes, ws);
-    else hipLaunchKernelGGL(k_clip, dim3(blocks), dim3(BLOCK), 0, s, d_passes, ws);
+    else hipLaunchKernelGGL(k_clip, dim3(4 * blocks), dim3(CLIP_LANES), 0, s, d_passes, ws);
     return hipGetLastError();
 }
 hipError_t launch_scan(const Workspace& ws, hipStream_t s) {
