@@ -3,9 +3,9 @@
 // One launch sequence renders any number of passes (a pass is one
 // graphics_draw_triangle call, SR/graphics.cpp:378-407):
 //
-//   k_vertex   one vertex per lane: vertex program, clip-space trivial-accept
+//   k_vertex   VPT vertices per lane: vertex program, clip-space trivial-accept
 //              bit, perspective divide and 1/w          (SR/IShader.cpp, SR/graphics.cpp:132-134, 316-337)
-//   k_setup<0> one face per lane: back-face cull, viewport, bbox, degenerate
+//   k_setup<0> FPT faces per lane: back-face cull, viewport, bbox, degenerate
 //              test of trivially accepted faces; counts (triangle, tile)
 //              pairs; queues the other faces for clipping
 //   k_clip     clips the queued faces (SR/graphics.cpp:136-161) into polygons
@@ -13,7 +13,7 @@
 //   k_scan     exclusive scan of the per-tile counts
 //   k_setup<1> same set-up; scatters triangle ids into the per-tile bins
 //   k_clip_scatter  scatters the clipped fan triangles
-//   k_raster   one workgroup per 32x32 tile: exact per-pixel coverage and
+//   k_raster   one workgroup per 16x16 tile: exact per-pixel coverage and
 //              depth (SR/graphics.cpp:222-233, 349-359) resolved in LDS with
 //              64-bit (depth, submission order) keys, then one fragment
 //              program per covered pixel (SR/graphics.cpp:362-373)
@@ -151,38 +151,55 @@ __global__ __launch_bounds__(BLOCK) void k_vertex(const PassDesc* passes, int np
     if (gtid < STATUS_WORDS) ws.status[gtid] = 0;
 
     const PassDesc& P = passes[find_pass<0>(passes, npasses, blockIdx.x)];
-    int v = (blockIdx.x - P.vblock0) * BLOCK + threadIdx.x;
-    bool vis = false;
-    if (v < P.nverts) {
-        const DevUniforms& u = *P.u;
-        F3 p = f3(P.px[v], P.py[v], P.pz[v]);
-        bool need_n = P.kind != SHADER_SHADOW && P.kind != SHADER_TEXTURE;
-        F3 n = need_n ? f3(P.nx[v], P.ny[v], P.nz[v]) : f3(0.f, 0.f, 0.f);
-        float tu = P.tu[v], tv = P.tv[v];
-        float clip[4], vary[8];
-        vertex_program(P.kind, u, p, n, tu, tv, clip, vary);
-        // is_vertex_visible (SR/graphics.cpp:132-134)
-        vis = fabsf(clip[0]) <= clip[3] && fabsf(clip[1]) <= clip[3] && fabsf(clip[2]) <= clip[3];
-        // clip_pos / clip_pos[3] and 1 / clip_pos[3] (SR/graphics.cpp:317, 336)
-        float4 r;
-        r.w = 1 / clip[3];
-        r.z = clip[2] / clip[3];
-        r.y = clip[1] / clip[3];
-        r.x = clip[0] / clip[3];
-        P.vr[v] = r;
-        if (P.nvary == 8) {
-            float4* d = reinterpret_cast<float4*>(P.vs + (size_t)v * 8);
-            d[0] = make_float4(vary[0], vary[1], vary[2], vary[3]);
-            d[1] = make_float4(vary[4], vary[5], vary[6], vary[7]);
-        } else if (P.nvary == 2) {
-            reinterpret_cast<float2*>(P.vs)[v] = make_float2(vary[0], vary[1]);
-        } else {
-            P.vs[v] = vary[0];
+    const DevUniforms& u = *P.u;
+    int base = (blockIdx.x - P.vblock0) * BLOCK * VPT;
+    bool need_n = P.kind != SHADER_SHADOW && P.kind != SHADER_TEXTURE;
+    // all loads of the thread's VPT vertices first (coalesced SoA streams)
+    F3 pos[VPT], nrm[VPT];
+    float tu[VPT], tv[VPT];
+#pragma unroll
+    for (int k = 0; k < VPT; k++) {
+        int v = base + k * BLOCK + threadIdx.x;
+        pos[k] = f3(0.f, 0.f, 0.f);
+        nrm[k] = f3(0.f, 0.f, 0.f);
+        tu[k] = tv[k] = 0.f;
+        if (v < P.nverts) {
+            pos[k] = f3(P.px[v], P.py[v], P.pz[v]);
+            if (need_n) nrm[k] = f3(P.nx[v], P.ny[v], P.nz[v]);
+            tu[k] = P.tu[v];
+            tv[k] = P.tv[v];
         }
     }
-    uint64_t bits = __ballot(vis);
-    int wave_first = (blockIdx.x - P.vblock0) * BLOCK + (threadIdx.x & ~63);
-    if ((threadIdx.x & 63) == 0 && wave_first < P.nverts) P.vis[wave_first >> 6] = bits;
+#pragma unroll
+    for (int k = 0; k < VPT; k++) {
+        int v = base + k * BLOCK + threadIdx.x;
+        bool vis = false;
+        if (v < P.nverts) {
+            float clip[4], vary[8];
+            vertex_program(P.kind, u, pos[k], nrm[k], tu[k], tv[k], clip, vary);
+            // is_vertex_visible (SR/graphics.cpp:132-134)
+            vis = fabsf(clip[0]) <= clip[3] && fabsf(clip[1]) <= clip[3] && fabsf(clip[2]) <= clip[3];
+            // clip_pos / clip_pos[3] and 1 / clip_pos[3] (SR/graphics.cpp:317, 336)
+            float4 r;
+            r.w = 1 / clip[3];
+            r.z = clip[2] / clip[3];
+            r.y = clip[1] / clip[3];
+            r.x = clip[0] / clip[3];
+            P.vr[v] = r;
+            if (P.nvary == 8) {
+                float4* d = reinterpret_cast<float4*>(P.vs + (size_t)v * 8);
+                d[0] = make_float4(vary[0], vary[1], vary[2], vary[3]);
+                d[1] = make_float4(vary[4], vary[5], vary[6], vary[7]);
+            } else if (P.nvary == 2) {
+                reinterpret_cast<float2*>(P.vs)[v] = make_float2(vary[0], vary[1]);
+            } else {
+                P.vs[v] = vary[0];
+            }
+        }
+        uint64_t bits = __ballot(vis);
+        int wave_first = base + k * BLOCK + (threadIdx.x & ~63);
+        if ((threadIdx.x & 63) == 0 && wave_first < P.nverts) P.vis[wave_first >> 6] = bits;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -380,37 +397,55 @@ __device__ __forceinline__ void emit_tiles(const PassDesc& P, const Workspace& w
     }
 }
 
-// One face per lane.  Trivially accepted faces are set up from the shared
+// FPT faces per lane.  Trivially accepted faces are set up from the shared
 // post-transform vertices; the others are queued for k_clip.
 template <int SCATTER>
 __global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* passes, int npasses, Workspace ws) {
     int pass = find_pass<1>(passes, npasses, blockIdx.x);
     const PassDesc& P = passes[pass];
-    int f = (blockIdx.x - P.fblock0) * BLOCK + threadIdx.x;
-    bool valid = f < P.nfaces;
-    bool trivial = false;
-    int i0 = 0, i1 = 0, i2 = 0;
-    if (valid) {
-        i0 = P.idx[3 * f];
-        i1 = P.idx[3 * f + 1];
-        i2 = P.idx[3 * f + 2];
-        trivial = vis_bit(P.vis, i0) && vis_bit(P.vis, i1) && vis_bit(P.vis, i2);
-    }
-    if (!SCATTER) {
-        bool clipq = valid && !trivial;
-        uint32_t q = wave_reserve(&ws.status[STATUS_CLIPQ], 0, clipq);
-        if (clipq && q < ws.clipq_cap) ws.clip_queue[q] = ((uint64_t)(uint32_t)pass << 32) | (uint32_t)f;
-        if (P.stats) {
-            uint64_t nv = __popcll(__ballot(valid)), nc = __popcll(__ballot(clipq));
-            if (__lane_id() == 0) {
-                stat_add(P, ST_FACES, nv);
-                stat_add(P, ST_CLIPPED, nc);
-            }
+    int base = (blockIdx.x - P.fblock0) * BLOCK * FPT;
+    int ids[FPT][3];
+    bool valid[FPT], trivial[FPT];
+#pragma unroll
+    for (int k = 0; k < FPT; k++) {
+        int f = base + k * BLOCK + threadIdx.x;
+        valid[k] = f < P.nfaces;
+        ids[k][0] = ids[k][1] = ids[k][2] = 0;
+        if (valid[k]) {
+            ids[k][0] = P.idx[3 * f];
+            ids[k][1] = P.idx[3 * f + 1];
+            ids[k][2] = P.idx[3 * f + 2];
         }
     }
-    TriS t;
-    bool ok = trivial && tri_setup(P.vr[i0], P.vr[i1], P.vr[i2], P.width, P.height, t);
-    emit_tiles<SCATTER>(P, ws, ok, t, (uint32_t)f);
+    float4 va[FPT], vb[FPT], vc[FPT];
+#pragma unroll
+    for (int k = 0; k < FPT; k++) {
+        trivial[k] = valid[k] && vis_bit(P.vis, ids[k][0]) && vis_bit(P.vis, ids[k][1]) && vis_bit(P.vis, ids[k][2]);
+        if (trivial[k]) {
+            va[k] = P.vr[ids[k][0]];
+            vb[k] = P.vr[ids[k][1]];
+            vc[k] = P.vr[ids[k][2]];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < FPT; k++) {
+        int f = base + k * BLOCK + threadIdx.x;
+        if (!SCATTER) {
+            bool clipq = valid[k] && !trivial[k];
+            uint32_t q = wave_reserve(&ws.status[STATUS_CLIPQ], 0, clipq);
+            if (clipq && q < ws.clipq_cap) ws.clip_queue[q] = ((uint64_t)(uint32_t)pass << 32) | (uint32_t)f;
+            if (P.stats) {
+                uint64_t nv = __popcll(__ballot(valid[k])), nc = __popcll(__ballot(clipq));
+                if (__lane_id() == 0) {
+                    stat_add(P, ST_FACES, nv);
+                    stat_add(P, ST_CLIPPED, nc);
+                }
+            }
+        }
+        TriS t;
+        bool ok = trivial[k] && tri_setup(va[k], vb[k], vc[k], P.width, P.height, t);
+        emit_tiles<SCATTER>(P, ws, ok, t, (uint32_t)f);
+    }
 }
 
 // Clips the queued faces (count phase): one wave per workgroup, grid-stride

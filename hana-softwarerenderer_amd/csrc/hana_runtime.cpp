@@ -204,7 +204,8 @@ int build_plan(hana_ctx* c, std::vector<PassSpec> specs, Plan& pl) {
         d.width = t->w; d.height = t->h;
         d.tiles_x = (t->w + TILE - 1) / TILE;
         d.tiles_y = (t->h + TILE - 1) / TILE;
-        size_t vpad = (V + BLOCK - 1) / BLOCK * BLOCK;
+        const size_t vchunk = (size_t)BLOCK * VPT, fchunk = (size_t)BLOCK * FPT;
+        size_t vpad = (V + vchunk - 1) / vchunk * vchunk;
         // slices (pointers filled after allocation): store offsets in the pointer fields
         d.vr = reinterpret_cast<float4*>(voff);
         d.vs = reinterpret_cast<float*>(vsoff);
@@ -214,9 +215,9 @@ int build_plan(hana_ctx* c, std::vector<PassSpec> specs, Plan& pl) {
         vsoff += vpad * (size_t)d.nvary;
         fo += (size_t)m->F;
         d.vblock0 = vb;
-        vb += (int)(vpad / BLOCK);
+        vb += (int)(vpad / vchunk);
         d.fblock0 = fb;
-        fb += (m->F + BLOCK - 1) / BLOCK;
+        fb += (int)(((size_t)m->F + fchunk - 1) / fchunk);
         d.tile0 = tile0;
         tile0 += d.tiles_x * d.tiles_y;
         d.tblock0 = tb[s.group];

@@ -15,10 +15,12 @@ enum {
     SHADER_SHADOW = 6
 };
 
-constexpr int TILE = 32;                   // screen tile edge (pixels)
-constexpr int TILE_SHIFT = 5;
+constexpr int TILE = 16;                   // screen tile edge (pixels)
+constexpr int TILE_SHIFT = 4;
 constexpr int TILE_PIX = TILE * TILE;
 constexpr int BLOCK = 256;                 // threads per workgroup (4 wave64)
+constexpr int VPT = 4;                     // vertices per thread in k_vertex
+constexpr int FPT = 4;                     // faces per thread in k_setup
 constexpr int POLY_MAX = 10;               // vertices of a clipped polygon (3 + 7 planes)
 constexpr int POLY_VARY = 8;               // varyings stored per clipped vertex
 constexpr uint32_t NO_WINNER = 0xFFFFFFFFu;
