@@ -1,0 +1,125 @@
+// hana_internal.h -- runtime objects shared by the C ABI implementation
+// files (hana_runtime.cpp, hana_server.cpp).  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/hana_gpu.h"
+#include "hana_types.h"
+
+struct hana_mesh {
+    int32_t V = 0, F = 0;
+    float* soa = nullptr;      // px | py | pz | nx | ny | nz | u | v, V floats each
+    int32_t* idx = nullptr;    // 3F
+};
+
+struct hana_texture {
+    int32_t w = 0, h = 0;
+    uint32_t* texels = nullptr;   // TGAColor bytes b,g,r,a per word
+};
+
+struct hana_target {
+    int32_t w = 0, h = 0;
+    uint32_t* color = nullptr;    // RGBA8 (R in the low byte)
+    float* depth = nullptr;
+    uint32_t* winner = nullptr;
+    bool pending_clear = false;
+    uint32_t clear_rgba = 0;
+    float clear_depth = 0;
+};
+
+namespace hana {
+namespace rt {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct PassSpec {
+    hana_mesh* mesh;
+    int kind;
+    hana_uniforms u;
+    hana_texture* diffuse;
+    hana_texture* normal;
+    hana_target* shadow;    // sampled by the fragment program
+    hana_target* target;
+    int group;              // raster dependency group (shadow passes first)
+};
+
+// Grow-only device workspace for launch sequences.
+struct Arena {
+    DevBuf descs, unis, vr, vs, vis, pof, tile_count, tile_offset, tile_cursor, bins;
+    DevBuf pool_vr, pool_vs, pool_face, pool_pass, pool_n, clip_queue, status, stats;
+    uint32_t bin_cap = 0, poly_cap = 0, clipq_cap = 0;
+};
+
+struct Plan {
+    std::vector<PassSpec> specs;
+    std::vector<PassDesc> descs;
+    std::vector<DevUniforms> unis;
+    int vblocks = 0, fblocks = 0;
+    int ngroups = 0;
+    int group_first[2] = {0, 0}, group_count[2] = {0, 0}, group_blocks[2] = {0, 0};
+    size_t total_v = 0, total_vs = 0, total_vis = 0, total_f = 0;
+    int total_tiles = 0;
+    Workspace ws{};
+};
+
+constexpr int kRing = 32;
+
+}  // namespace rt
+}  // namespace hana
+
+struct hana_ctx {
+    using Arena = hana::rt::Arena;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool stats_enabled = false;
+    Arena arena;
+    // pinned staging ring for per-call descriptor uploads
+    void* ring_host[hana::rt::kRing] = {};
+    hipEvent_t ring_event[hana::rt::kRing] = {};
+    bool ring_used[hana::rt::kRing] = {};
+    size_t ring_bytes = 0;
+    int ring_next = 0;
+    // status of the last launch sequence
+    uint32_t* status_host = nullptr;   // pinned, STATUS_WORDS
+    uint64_t* stats_host = nullptr;    // pinned, per pass
+    int stats_passes = 0;
+    bool status_pending = false;
+    // optional per-stage timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+};
+
+struct hana_batch {
+    std::vector<hana::rt::PassSpec> specs;
+    hana::rt::Plan plan;
+    hana::rt::Arena arena;
+    bool finalized = false;
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace hana {
+namespace rt {
+
+int fail(hana_ctx* c, int code, const std::string& msg);
+int hip_fail(hana_ctx* c, hipError_t e, const char* what);
+int ensure(hana_ctx* c, DevBuf& b, size_t bytes);
+int materialize_clear(hana_ctx* c, hana_target* t);
+int check_status(hana_ctx* c);
+// Enqueues the launch sequence of a set of passes on the context's stream.
+int run_passes(hana_ctx* c, std::vector<PassSpec> specs);
+bool valid_kind(int k);
+
+}  // namespace rt
+}  // namespace hana

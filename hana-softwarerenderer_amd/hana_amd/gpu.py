@@ -51,6 +51,33 @@ class PassStats(C.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("lanes", C.c_int32), ("max_batch_frames", C.c_int32), ("max_batch_pixels", C.c_int64),
+                ("hbm_budget_bytes", C.c_int64)]
+
+
+class Request(C.Structure):
+    _fields_ = [("mesh", _P), ("diffuse", _P), ("normal", _P), ("shader", C.c_int32), ("enable_shadow", C.c_int32),
+                ("width", C.c_int32), ("height", C.c_int32), ("shadow_width", C.c_int32),
+                ("shadow_height", C.c_int32), ("cam_pos", C.c_float * 3), ("cam_target", C.c_float * 3),
+                ("light_pos", C.c_float * 3), ("pad_", C.c_int32), ("user_tag", C.c_uint64)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("user_tag", C.c_uint64), ("status", C.c_int32), ("width", C.c_int32),
+                ("height", C.c_int32), ("batch_frames", C.c_int32), ("rgba", C.POINTER(C.c_uint8)),
+                ("t_submit_ms", C.c_double), ("t_launch_ms", C.c_double), ("t_done_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64), ("batches", C.c_int64),
+                ("retries", C.c_int64), ("inflight_bytes", C.c_int64), ("peak_inflight_bytes", C.c_int64),
+                ("busy_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: (float(getattr(self, k)) if k == "busy_ms" else int(getattr(self, k))) for k, _ in self._fields_}
+
+
 DECLS = [
     ("hana_gpu_create", C.c_int, [C.c_int, C.POINTER(_P)]),
     ("hana_gpu_destroy", C.c_int, [_P]),
@@ -78,6 +105,13 @@ DECLS = [
     ("hana_batch_finalize", C.c_int, [_P, _P, C.c_int]),
     ("hana_batch_run", C.c_int, [_P, _P]),
     ("hana_batch_destroy", C.c_int, [_P, _P]),
+    ("hana_server_create", C.c_int, [_P, C.POINTER(ServerConfig), C.POINTER(_P)]),
+    ("hana_server_submit", C.c_int, [_P, C.POINTER(Request), C.POINTER(C.c_uint64)]),
+    ("hana_server_poll", C.c_int, [_P, C.POINTER(Completion), C.c_int, C.POINTER(C.c_int), C.c_int]),
+    ("hana_server_release", C.c_int, [_P, C.c_uint64]),
+    ("hana_server_get_stats", C.c_int, [_P, C.POINTER(ServerStats)]),
+    ("hana_server_now_ms", C.c_double, [_P]),
+    ("hana_server_destroy", C.c_int, [_P]),
     ("hana_build_uniforms", C.c_int, [_FP, _FP, C.c_float, _FP, _FP, _FP, _FP, C.c_int32, C.c_int32, C.c_int32,
                                       C.POINTER(Uniforms)]),
 ]
@@ -305,3 +339,56 @@ class SceneOnDevice:
 
     def read(self):
         return self.ctx.read(self.target, self.scene.width, self.scene.height)
+
+
+class Server:
+    """Continuous-batching render server (hana_server_*)."""
+
+    def __init__(self, ctx: Context, lanes=2, max_batch_frames=64, max_batch_pixels=64 << 20,
+                 hbm_budget_bytes=200 << 30):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        cfg = ServerConfig(lanes, max_batch_frames, max_batch_pixels, hbm_budget_bytes)
+        h = _P()
+        ctx.check(self.lib.hana_server_create(ctx.h, C.byref(cfg), C.byref(h)), "server_create")
+        self.h = h
+
+    def submit(self, mesh, shader, shadow, width, height, cam_pos, light_pos, diffuse=None, normal=None,
+               cam_target=(0.0, 0.0, 0.0), shadow_wh=(0, 0), tag=0) -> int:
+        r = Request(mesh, diffuse, normal, int(shader), int(shadow), int(width), int(height), int(shadow_wh[0]),
+                    int(shadow_wh[1]), (C.c_float * 3)(*cam_pos), (C.c_float * 3)(*cam_target),
+                    (C.c_float * 3)(*light_pos), 0, int(tag))
+        rid = C.c_uint64()
+        rc = self.lib.hana_server_submit(self.h, C.byref(r), C.byref(rid))
+        if rc:
+            raise HanaError(rc, "server_submit")
+        return int(rid.value)
+
+    def poll(self, max_n=256, wait_ms=10):
+        arr = (Completion * max_n)()
+        n = C.c_int()
+        rc = self.lib.hana_server_poll(self.h, arr, max_n, C.byref(n), int(wait_ms))
+        if rc:
+            raise HanaError(rc, "server_poll")
+        return [arr[i] for i in range(n.value)]
+
+    @staticmethod
+    def image(c: Completion) -> np.ndarray:
+        n = c.width * c.height * 4
+        return np.ctypeslib.as_array(c.rgba, shape=(n,)).reshape(c.height, c.width, 4).copy()
+
+    def release(self, rid: int):
+        self.lib.hana_server_release(self.h, C.c_uint64(rid))
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        self.lib.hana_server_get_stats(self.h, C.byref(st))
+        return st.as_dict()
+
+    def now_ms(self) -> float:
+        return float(self.lib.hana_server_now_ms(self.h))
+
+    def close(self):
+        if self.h:
+            self.lib.hana_server_destroy(self.h)
+            self.h = None

@@ -591,3 +591,53 @@ def small_scene(kind: int = NORMALMAP, shadow: bool = False, width: int = 160, h
     d, n = texture_pair(tex, tex, 900 + seed)
     return Scene(mesh, width, height, kind, shadow, cam, random_light(rng), d, n,
                  shadow_size=shadow_size, name=f"small/{seed}")
+
+
+@dataclass
+class ServeWorkload:
+    """Config 4: resident meshes and a Poisson stream of render requests."""
+    meshes: list
+    textures: list
+    requests: list          # dicts: t_ms, mesh, shader, shadow, width, height, cam_pos, light_pos, tex
+
+
+def config4_workload(n_meshes: int = 64, tri_lo: int = 5000, tri_hi: int = 500_000, rate_per_s: float = 2000.0,
+                     seconds: float = 60.0, px_lo: int = 256 * 256, px_hi: int = 1920 * 1080,
+                     max_requests: int | None = None) -> ServeWorkload:
+    """64 device-resident meshes (log-uniform 5k-500k triangles), Poisson
+    arrivals at rate_per_s; each request a random orbit camera and light over
+    one mesh at a log-uniform pixel count in [256^2, 1920x1080] with random
+    aspect, Blinn or NormalMap, shadow on or off."""
+    rng = config_rng(3)
+    texs = [texture_pair(1024, 1024, 61 + k) for k in range(4)]
+    meshes = []
+    for i in range(n_meshes):
+        t = int(log_uniform(rng, tri_lo, tri_hi))
+        if rng.uniform() < 0.75:
+            meshes.append(sphere_with_tris(t, 0.05, seed=int(rng.integers(1 << 30))))
+        else:
+            m = box_with_tris(t)
+            meshes.append(place(m, rotation_matrix(rng), 0.8, np.zeros(3)))
+    reqs = []
+    t = 0.0
+    limit = max_requests if max_requests is not None else 1 << 62
+    while len(reqs) < limit:
+        t += rng.exponential(1000.0 / rate_per_s)
+        if t > seconds * 1000.0:
+            break
+        npx = log_uniform(rng, px_lo, px_hi)
+        aspect = log_uniform(rng, 0.5, 2.0)
+        w = max(16, min(4096, int(round(math.sqrt(npx * aspect)))))
+        h = max(16, min(4096, int(round(math.sqrt(npx / aspect)))))
+        reqs.append({"t_ms": t, "mesh": int(rng.integers(n_meshes)),
+                     "shader": BLINN if rng.uniform() < 0.5 else NORMALMAP, "shadow": bool(rng.uniform() < 0.5),
+                     "width": w, "height": h, "cam_pos": orbit_camera(rng, 2.2, 3.2), "light_pos": random_light(rng),
+                     "tex": int(rng.integers(len(texs)))})
+    return ServeWorkload(meshes, texs, reqs)
+
+
+def request_scene(wl: ServeWorkload, req: dict) -> Scene:
+    """The DrawModel::draw frame a request renders."""
+    d, n = wl.textures[req["tex"]]
+    return Scene(wl.meshes[req["mesh"]], req["width"], req["height"], req["shader"], req["shadow"],
+                 tuple(req["cam_pos"]), tuple(req["light_pos"]), d, n, name=f"C4/{req['t_ms']:.3f}")

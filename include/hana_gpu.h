@@ -165,6 +165,64 @@ int hana_batch_finalize(hana_ctx* ctx, hana_batch* b, int use_graph);
 int hana_batch_run(hana_ctx* ctx, hana_batch* b);
 int hana_batch_destroy(hana_ctx* ctx, hana_batch* b);
 
+/* ---- serving: continuous batching of render requests ----
+ * No single reference counterpart: the reference renders one scene per
+ * window frame (SR/main.cpp:87-156, DrawModel::draw SR/scene.h:53-99).  A
+ * server keeps meshes and textures resident, admits requests while the
+ * in-flight HBM footprint stays within hbm_budget_bytes, forms a batch of
+ * whatever is admitted whenever one of its lanes (HIP streams) is idle, and
+ * runs lanes concurrently.  Every request is a DrawModel::draw frame on
+ * cleared targets; its colour image is copied back before completion. */
+typedef struct hana_server hana_server;
+
+typedef struct hana_server_config {
+    int32_t lanes;                /* concurrent batches (streams), >= 1 */
+    int32_t max_batch_frames;     /* frames per batch */
+    int64_t max_batch_pixels;     /* frame + shadow pixels per batch */
+    int64_t hbm_budget_bytes;     /* in-flight device footprint limit */
+} hana_server_config;
+
+typedef struct hana_request {
+    hana_mesh* mesh;
+    hana_texture* diffuse;
+    hana_texture* normal;
+    int32_t shader;               /* HANA_SHADER_* */
+    int32_t enable_shadow;
+    int32_t width, height;
+    int32_t shadow_width, shadow_height;   /* 0: frame-sized, as the reference */
+    float cam_pos[3];
+    float cam_target[3];
+    float light_pos[3];
+    int32_t pad_;
+    uint64_t user_tag;
+} hana_request;
+
+typedef struct hana_completion {
+    uint64_t id;
+    uint64_t user_tag;
+    int32_t status;               /* HANA_OK or a negative code */
+    int32_t width, height;
+    int32_t batch_frames;         /* size of the batch it ran in */
+    const uint8_t* rgba;          /* W*H*4 image, valid until hana_server_release */
+    double t_submit_ms, t_launch_ms, t_done_ms;   /* server clock */
+} hana_completion;
+
+typedef struct hana_server_stats {
+    int64_t submitted, completed, rejected, batches, retries;
+    int64_t inflight_bytes, peak_inflight_bytes;
+    double busy_ms;               /* time with at least one batch in flight */
+} hana_server_stats;
+
+int hana_server_create(hana_ctx* ctx, const hana_server_config* cfg, hana_server** out);
+/* Thread-safe; returns the request id. */
+int hana_server_submit(hana_server* s, const hana_request* r, uint64_t* id);
+/* Up to max completions; waits up to wait_ms for the first one. */
+int hana_server_poll(hana_server* s, hana_completion* out, int max, int* n, int wait_ms);
+int hana_server_release(hana_server* s, uint64_t id);
+int hana_server_get_stats(hana_server* s, hana_server_stats* out);
+double hana_server_now_ms(hana_server* s);
+int hana_server_destroy(hana_server* s);
+
 /* ---- host uniform math ---- */
 /* ShaderData as DrawModel::draw fills it for Camera(cam_pos, cam_target,
  * cam_aspect), a light GameObject at light_pos, the model transform
