@@ -57,6 +57,153 @@ def cpu_leg(scene, seconds):
             "target": tgt}
 
 
+def cpu_frame_counts(scenes, label):
+    """CPU leg for a multi-scene workload: renders every scene once with the
+    oracle on one core; returns per-scene fragment counts and the time."""
+    from pyoracle import CPURenderer
+    o = CPURenderer("oracle")
+    counts = []
+    t0 = time.perf_counter()
+    for sc in scenes:
+        tgt, sm, st_s, st_m = o.render_passes(sc)
+        counts.append(st_m["fragments"] + (st_s["fragments"] if st_s else 0))
+    dt = time.perf_counter() - t0
+    log(f"cpu {label}: {len(scenes)} scenes in {dt:.1f} s, {sum(counts)} fragments")
+    return counts, dt
+
+
+def bench_batched(ctx, args):
+    """256 independent scenes (1-8 displaced spheres/boxes, 1k-200k tris,
+    128^2-1024^2 px, mixed shaders and shadows): per-frame launches against one
+    captured graph for the whole batch."""
+    from hana_amd import scenegen as sg
+    from hana_amd.gpu import Batch, SceneOnDevice
+    t0 = time.perf_counter()
+    scenes = sg.config3_scenes()
+    log(f"batched: generated {len(scenes)} scenes in {time.perf_counter() - t0:.1f} s")
+    frags = None
+    cpu = None
+    if not args.no_cpu:
+        counts, cpu_dt = cpu_frame_counts(scenes, "batched")
+        frags = sum(counts)
+        cpu = {"value": round(frags / cpu_dt / 1e6, 4), "unit": "Mfrag/s", "cores": 1, "kind": "port",
+               "sample": f"all {len(scenes)} scenes once with the oracle, single-threaded, {cpu_dt:.1f} s"}
+    devs = [SceneOnDevice(ctx, s) for s in scenes]
+    # per-frame: one launch sequence per scene
+    for d in devs:
+        d.clear()
+        d.render()
+    ctx.sync()
+    reps = args.batch_reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for d in devs:
+            d.clear()
+            d.render()
+    ctx.sync()
+    t_frame = (time.perf_counter() - t0) / reps
+    per_frame_out = [d.read()[0] for d in devs[:8]]
+    batch = Batch(ctx)
+    for d in devs:
+        batch.add(d.desc())
+    batch.finalize(use_graph=True)
+    batch.run()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.run()
+    ctx.sync()
+    t_batch = (time.perf_counter() - t0) / reps
+    same = all(np.array_equal(d.read()[0], c) for d, c in zip(devs[:8], per_frame_out))
+    batch.close()
+    out = {"workload": f"{len(scenes)} scenes, {sum(s.mesh.nfaces for s in scenes)} tris, "
+                       f"{sum(s.width * s.height for s in scenes)} px, {sum(s.shadow for s in scenes)} with shadow",
+           "per_frame_ms_per_batch": round(t_frame * 1e3, 3), "batched_ms_per_batch": round(t_batch * 1e3, 3),
+           "per_frame_frames_per_s": round(len(scenes) / t_frame, 1),
+           "batched_frames_per_s": round(len(scenes) / t_batch, 1),
+           "speedup": round(t_frame / t_batch, 2), "fragments_per_batch": frags,
+           "per_frame_Mfrag_s": round(frags / t_frame / 1e6, 2) if frags else None,
+           "batched_Mfrag_s": round(frags / t_batch / 1e6, 2) if frags else None,
+           "batched_equals_per_frame": same, "cpu_baseline": cpu}
+    log("batched:", json.dumps(out))
+    return out
+
+
+def bench_serving(ctx, args):
+    """Poisson arrivals over 64 resident meshes (5k-500k tris) at 256^2 to
+    1920x1080, continuous batching within an HBM budget; p50/p99 latency from
+    submission to the colour image in host memory."""
+    import threading
+    from hana_amd import scenegen as sg
+    from hana_amd.gpu import Server
+    t0 = time.perf_counter()
+    wl = sg.config4_workload(rate_per_s=args.serve_rate, seconds=args.serve_seconds)
+    log(f"serving: {len(wl.requests)} requests over {args.serve_seconds} s at {args.serve_rate}/s, "
+        f"{sum(m.nfaces for m in wl.meshes)} resident tris (generated in {time.perf_counter() - t0:.1f} s)")
+    frags_per_req = None
+    if not args.no_cpu and args.serve_cpu_sample > 0:
+        sample = wl.requests[: args.serve_cpu_sample]
+        counts, cpu_dt = cpu_frame_counts([sg.request_scene(wl, r) for r in sample], "serving sample")
+        frags_per_req = sum(counts) / len(counts)
+        cpu_req_s = len(sample) / cpu_dt
+    meshes = [ctx.mesh(m) for m in wl.meshes]
+    texs = [(ctx.texture(d), ctx.texture(n)) for d, n in wl.textures]
+    server = Server(ctx, lanes=args.serve_lanes, max_batch_frames=args.serve_max_batch,
+                    hbm_budget_bytes=int(args.serve_budget_gb * (1 << 30)))
+    lat = []
+    batch_sizes = []
+    errors = [0]
+    done = threading.Event()
+
+    def poller():
+        got = 0
+        while got < len(wl.requests):
+            cs = server.poll(256, wait_ms=20)
+            for c in cs:
+                if c.status != 0:
+                    errors[0] += 1
+                else:
+                    lat.append(c.t_done_ms - c.t_submit_ms)
+                    batch_sizes.append(c.batch_frames)
+                server.release(int(c.id))
+            got += len(cs)
+            if done.is_set() and not cs and got >= len(wl.requests):
+                break
+    th = threading.Thread(target=poller, daemon=True)
+    th.start()
+    start = time.perf_counter()
+    for i, r in enumerate(wl.requests):
+        target_t = start + r["t_ms"] / 1000.0
+        while True:
+            dt = target_t - time.perf_counter()
+            if dt <= 0:
+                break
+            time.sleep(min(dt, 0.0005))
+        d, n = texs[r["tex"]]
+        server.submit(meshes[r["mesh"]], r["shader"], r["shadow"], r["width"], r["height"], r["cam_pos"],
+                      r["light_pos"], d, n, tag=i)
+    done.set()
+    th.join(timeout=120)
+    wall = time.perf_counter() - start
+    st = server.stats()
+    server.close()
+    lat_a = np.array(lat) if lat else np.zeros(1)
+    out = {"workload": f"{len(wl.requests)} requests, Poisson {args.serve_rate}/s for {args.serve_seconds} s, "
+                       f"64 meshes 5k-500k tris, 256^2-1920x1080, half shadowed, budget {args.serve_budget_gb} GB",
+           "completed": len(lat), "errors": errors[0], "requests_per_s": round(len(lat) / wall, 1),
+           "p50_ms": round(float(np.percentile(lat_a, 50)), 3), "p99_ms": round(float(np.percentile(lat_a, 99)), 3),
+           "max_ms": round(float(lat_a.max()), 3), "mean_batch_frames": round(float(np.mean(batch_sizes)), 2)
+           if batch_sizes else None, "gpu_busy_frac": round(st["busy_ms"] / (wall * 1e3), 3),
+           "batches": st["batches"], "peak_inflight_GB": round(st["peak_inflight_bytes"] / 2 ** 30, 3)}
+    if frags_per_req:
+        out["est_Mfrag_s"] = round(frags_per_req * len(lat) / wall / 1e6, 2)
+        out["cpu_baseline"] = {"value": round(cpu_req_s, 3), "unit": "requests/s", "cores": 1, "kind": "port",
+                               "sample": f"first {args.serve_cpu_sample} requests with the oracle, single-threaded"}
+        out["fragments_per_request_sample"] = round(frags_per_req, 1)
+    log("serving:", json.dumps(out))
+    return out
+
+
 def alg_bytes(scene):
     """Algorithmic HBM bytes of one frame, split by stage:
     a2v 32 B and indices 12 B per vertex/triangle per pass, colour+depth 8 B
@@ -84,6 +231,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--skip-batched", action="store_true")
+    ap.add_argument("--skip-serving", action="store_true")
+    ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-lanes", type=int, default=3)
+    ap.add_argument("--serve-max-batch", type=int, default=64)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serve-cpu-sample", type=int, default=40)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy (scales rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -186,6 +342,10 @@ def main():
                                          f"{cpu['sec_per_frame']:.2f} s/frame",
                                "frames_per_s": round(1.0 / cpu["sec_per_frame"], 4)}
         res["gpu_over_cpu"] = round(value / cpu_mfrag, 1) if value else None
+    if not args.skip_batched:
+        res["batched"] = bench_batched(ctx, args)
+    if not args.skip_serving:
+        res["serving"] = bench_serving(ctx, args)
     print(json.dumps(res), flush=True)
     ctx.close()
 
