@@ -62,6 +62,8 @@ struct Plan {
     std::vector<PassSpec> specs;
     std::vector<PassDesc> descs;
     std::vector<DevUniforms> unis;
+    // per-pass workspace slice offsets (vertices, varying floats, faces)
+    std::vector<size_t> v_off, vs_off, f_off;
     int vblocks = 0, fblocks = 0;
     int ngroups = 0;
     int group_first[2] = {0, 0}, group_count[2] = {0, 0}, group_blocks[2] = {0, 0};

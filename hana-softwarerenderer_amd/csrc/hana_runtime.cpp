@@ -113,11 +113,10 @@ int build_plan(hana_ctx* c, std::vector<PassSpec> specs, Plan& pl) {
         d.tiles_y = (t->h + TILE - 1) / TILE;
         const size_t vchunk = (size_t)BLOCK * VPT, fchunk = (size_t)BLOCK * FPT;
         size_t vpad = (V + vchunk - 1) / vchunk * vchunk;
-        // slices (pointers filled after allocation): store offsets in the pointer fields
-        d.vr = reinterpret_cast<float4*>(voff);
-        d.vs = reinterpret_cast<float*>(vsoff);
-        d.vis = reinterpret_cast<uint64_t*>(voff / 64);
-        d.poly_of_face = reinterpret_cast<int32_t*>(fo);
+        // slices: pointers are resolved by bind_plan once the arena exists
+        pl.v_off.push_back(voff);
+        pl.vs_off.push_back(vsoff);
+        pl.f_off.push_back(fo);
         voff += vpad;
         vsoff += vpad * (size_t)d.nvary;
         fo += (size_t)m->F;
@@ -195,9 +194,7 @@ int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_p
     if ((r = ensure(c, A.stats, sizeof(uint64_t) * ST_WORDS * std::max(n, 1)))) return r;
     for (int i = 0; i < n; i++) {
         PassDesc& d = pl.descs[i];
-        size_t voff = reinterpret_cast<size_t>(d.vr);
-        size_t vsoff = reinterpret_cast<size_t>(d.vs);
-        size_t fo = reinterpret_cast<size_t>(d.poly_of_face);
+        size_t voff = pl.v_off[i], vsoff = pl.vs_off[i], fo = pl.f_off[i];
         d.vr = A.vr.as<float4>() + voff;
         d.vs = A.vs.as<float>() + vsoff;
         d.vis = A.vis.as<uint64_t>() + voff / 64;
