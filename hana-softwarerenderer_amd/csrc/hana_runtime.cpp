@@ -753,6 +753,13 @@ static void batch_clear_flags(hana_batch* b) {
     }
 }
 
+// The sequence itself clears every target it draws, so a clear still pending
+// from target creation or hana_target_clear is void: left pending, it would
+// wipe the rendered image at the next read.
+static void batch_drop_pending(hana_batch* b) {
+    for (auto& s : b->specs) s.target->pending_clear = false;
+}
+
 int hana_batch_finalize(hana_ctx* c, hana_batch* b, int use_graph) {
     if (!c || !b) return HANA_E_INVALID;
     if (b->specs.empty()) return fail(c, HANA_E_INVALID, "hana_batch_finalize: empty batch");
@@ -776,6 +783,7 @@ int hana_batch_finalize(hana_ctx* c, hana_batch* b, int use_graph) {
         uint32_t st[STATUS_WORDS];
         HIP_TRY(c, hipMemcpyAsync(st, b->arena.status.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
+        batch_drop_pending(b);
         if (!st[STATUS_FLAGS]) break;
         min_bins = st[STATUS_BINS_NEEDED] + st[STATUS_BINS_NEEDED] / 8 + 1024;
         min_polys = st[STATUS_POLYS] + st[STATUS_POLYS] / 8 + 1024;
@@ -798,6 +806,7 @@ int hana_batch_finalize(hana_ctx* c, hana_batch* b, int use_graph) {
 
 int hana_batch_run(hana_ctx* c, hana_batch* b) {
     if (!c || !b || !b->finalized) return fail(c, HANA_E_INVALID, "hana_batch_run: batch not finalized");
+    batch_drop_pending(b);
     if (b->use_graph) {
         HIP_TRY(c, hipGraphLaunch(b->exec, c->stream));
         return HANA_OK;
