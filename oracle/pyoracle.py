@@ -3,7 +3,11 @@
 Two libraries expose the same entry points (hana_oracle.h):
   * liboracle.so          -- the C restatement, prefix ``ho_``;
   * _ref/libhanaref.so    -- the reference's own sources compiled by
-                             oracle/Makefile, prefix ``ref_``.
+                             oracle/Makefile, prefix ``ref_``;
+  * _ref/libhanaref_dropin.so -- the same, with graphics_draw_triangle taken
+                             from the MI355X drop-in (its ref_draw_model runs
+                             the reference's Model/IShader/RenderBuffer through
+                             libhana_gpu.so; every other entry point is CPU).
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use them.
 """
 from __future__ import annotations
@@ -16,6 +20,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libhanaref.so")
+DROPIN_SO = os.path.join(HERE, "_ref", "libhanaref_dropin.so")
 
 FLT_MAX = float(np.finfo(np.float32).max)
 
@@ -84,6 +89,20 @@ class HostTarget:
         return Target(self.width, self.height, _ptr(self.color), _ptr(self.depth))
 
 
+def write_obj(mesh, path) -> None:
+    """Wavefront OBJ the reference's parser reads (v / vt / vn, f v/t/n with
+    one index for all three); %.9g round-trips every float32 exactly."""
+    with open(path, "w") as f:
+        for p in np.asarray(mesh.pos, np.float32):
+            f.write("v %.9g %.9g %.9g\n" % tuple(float(x) for x in p))
+        for t in np.asarray(mesh.uv, np.float32):
+            f.write("vt %.9g %.9g 0\n" % tuple(float(x) for x in t))
+        for n in np.asarray(mesh.nrm, np.float32):
+            f.write("vn %.9g %.9g %.9g\n" % tuple(float(x) for x in n))
+        for a, b, c in np.asarray(mesh.idx, np.int64) + 1:
+            f.write(f"f {a}/{a}/{a} {b}/{b}/{b} {c}/{c}/{c}\n")
+
+
 def image_struct(img: np.ndarray | None) -> Image:
     if img is None:
         return Image(0, 0, 0, 0, None)
@@ -101,6 +120,8 @@ class CPURenderer:
             path, self.prefix, self.kind = ORACLE_SO, "ho_", "port"
         elif which == "reference":
             path, self.prefix, self.kind = REF_SO, "ref_", "reference"
+        elif which == "dropin":
+            path, self.prefix, self.kind = DROPIN_SO, "ref_", "dropin"
         else:
             raise ValueError(which)
         if not os.path.exists(path):
@@ -266,6 +287,18 @@ class CPURenderer:
         self._draw(C.byref(m), kind, C.byref(u), C.byref(d), C.byref(n), C.byref(s) if s is not None else None,
                    C.byref(t), C.byref(st), _ptr(win) if win is not None else None)
         return st.as_dict(), win
+
+    def draw_model(self, obj_path: str, kind, u: Uniforms, target: HostTarget, diffuse=None, normal=None,
+                   shadow: HostTarget | None = None) -> int:
+        """graphics_draw_triangle itself on a Model read from obj_path by the
+        reference's OBJ parser (reference and drop-in libraries only)."""
+        if self._draw_model is None:
+            raise RuntimeError(f"{self.path} has no draw_model entry point")
+        d, n = image_struct(diffuse), image_struct(normal)
+        t = target.struct()
+        s = shadow.struct() if shadow is not None else None
+        return self._draw_model(os.fsencode(obj_path), kind, C.byref(u), C.byref(d), C.byref(n),
+                                C.byref(s) if s is not None else None, C.byref(t))
 
     def render_frame(self, scene, u: Uniforms | None = None, target: HostTarget | None = None,
                      shadow_map: HostTarget | None = None):

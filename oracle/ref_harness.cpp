@@ -22,12 +22,25 @@
 //    SR/camera.cpp:11-14, as Camera::get_view_matrix/get_proj_matrix do.
 //  * DrawModel::draw (SR/scene.h:53-99) is restated in ref_build_uniforms and
 //    ref_render_frame with the reference's types and functions.
+//  * Built with -DHANA_DROPIN (oracle/_ref/libhanaref_dropin.so), the
+//    reference's graphics_draw_triangle is replaced by the MI355X drop-in, so
+//    ref_draw_model runs the reference's Model, shaders and RenderBuffer
+//    through libhana_gpu.so; every other entry point stays on the CPU.
 #include <cfloat>
 #include <cstring>
 #include <limits>
 
 #include "vector.cpp"   // first: its specialisations must precede first use
+#ifdef HANA_DROPIN
+// The drop-in build keeps the CPU face loop under another name and takes
+// graphics_draw_triangle from hana-softwarerenderer_amd/dropin/graphics_hana.cpp.
+#define graphics_draw_triangle graphics_draw_triangle_cpu
 #include "graphics.cpp"
+#undef graphics_draw_triangle
+void graphics_draw_triangle(DrawData* draw_data);
+#else
+#include "graphics.cpp"
+#endif
 #include "IShader.cpp"
 #include "renderbuffer.cpp"
 #include "color.cpp"
@@ -36,6 +49,9 @@
 #include "model.cpp"
 #include "gameobject.cpp"
 #include "scene.h"      // AMBIENT, LightColor (SR/scene.h:8-9); DrawModel is not instantiated
+#ifdef HANA_DROPIN
+#include "graphics_hana.cpp"
+#endif
 
 #include "hana_oracle.h"
 
