@@ -204,6 +204,65 @@ def bench_serving(ctx, args):
     return out
 
 
+def bench_micro(ctx, args):
+    """Micro-triangle stress: 8 jittered grid layers (20 M triangles of 1-4 px)
+    over 3840x2160 with exact depth ties between layer pairs, NormalMap with a
+    4096^2 shadow pre-pass; frames/s and shaded Mfrag/s as for the large frame."""
+    from hana_amd import scenegen as sg
+    from hana_amd.gpu import SceneOnDevice
+    t0 = time.perf_counter()
+    scene = sg.config5_scene()
+    log(f"micro: {scene.mesh.nfaces} tris, {scene.width}x{scene.height}, shadow {scene.shadow_wh} "
+        f"({time.perf_counter() - t0:.1f} s to generate)")
+    frags = None
+    cpu = None
+    if not args.no_cpu:
+        from pyoracle import CPURenderer
+        o = CPURenderer("oracle")
+        t0 = time.perf_counter()
+        tgt, sm, st_s, st_m = o.render_passes(scene)
+        cpu_dt = time.perf_counter() - t0
+        frags = st_m["fragments"] + st_s["fragments"]
+        log(f"cpu micro: 1 frame in {cpu_dt:.1f} s, {frags} fragments")
+        cpu = {"value": round(frags / cpu_dt / 1e6, 4), "unit": "Mfrag/s", "cores": 1, "kind": "port",
+               "sample": f"one whole frame with the oracle, single-threaded, {cpu_dt:.1f} s"}
+    dev = SceneOnDevice(ctx, scene)
+    for _ in range(2):
+        dev.clear()
+        dev.render()
+    ctx.sync()
+    steps = args.micro_steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dev.clear()
+        dev.render()
+    ctx.sync()
+    dt = (time.perf_counter() - t0) / steps
+    ctx.set_timing(True)
+    for _ in range(steps):
+        dev.clear()
+        dev.render()
+    stages = ctx.read_timing()
+    ctx.set_timing(False)
+    stage_ms = {k: round(v[0] / v[1], 5) if v[1] else 0.0 for k, v in stages.items()}
+    parity = None
+    if cpu is not None:
+        col, dep = dev.read()
+        d = np.abs(col[..., :3].astype(np.int16) - tgt.color[..., :3].astype(np.int16)).max(-1)
+        parity = {"depth_bit_exact": bool(np.array_equal(dep.view(np.uint32), tgt.depth.view(np.uint32))),
+                  "color_max_lsb": int(d.max()), "color_frac_le1": float((d <= 1).mean())}
+    _, frame_bytes = alg_bytes(scene)
+    out = {"workload": f"{scene.mesh.nfaces} tris in 8 layers, {scene.width}x{scene.height}, "
+                       f"shadow {scene.shadow_wh[0]}^2, NormalMap",
+           "ms_per_frame": round(dt * 1e3, 3), "frames_per_s": round(1.0 / dt, 2),
+           "fragments_per_frame": frags, "Mfrag_s": round(frags / dt / 1e6, 2) if frags else None,
+           "frame_alg_bytes": frame_bytes, "frame_achieved_GBps": round(frame_bytes / dt / 1e9, 2),
+           "stage_ms": stage_ms, "parity": parity, "cpu_baseline": cpu}
+    log("micro:", json.dumps(out))
+    del dev
+    return out
+
+
 def alg_bytes(scene):
     """Algorithmic HBM bytes of one frame, split by stage:
     a2v 32 B and indices 12 B per vertex/triangle per pass, colour+depth 8 B
@@ -240,6 +299,8 @@ def main():
     ap.add_argument("--serve-max-batch", type=int, default=64)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--serve-cpu-sample", type=int, default=40)
+    ap.add_argument("--skip-micro", action="store_true")
+    ap.add_argument("--micro-steps", type=int, default=5)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy (scales rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -346,6 +407,8 @@ def main():
         res["batched"] = bench_batched(ctx, args)
     if not args.skip_serving:
         res["serving"] = bench_serving(ctx, args)
+    if not args.skip_micro:
+        res["micro"] = bench_micro(ctx, args)
     print(json.dumps(res), flush=True)
     ctx.close()
 

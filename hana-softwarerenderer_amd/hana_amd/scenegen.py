@@ -451,11 +451,14 @@ class Scene:
         return self.shadow_size if self.shadow_size else (self.width, self.height)
 
 
-def random_light(rng: np.random.Generator, radius: float = 2.0 * math.sqrt(3.0)) -> tuple:
+def random_light(rng: np.random.Generator, radius: float = 2.0 * math.sqrt(3.0), front: bool = False) -> tuple:
     d = rng.normal(size=3)
     d /= np.linalg.norm(d)
     if d[1] < 0.2:             # keep the light above the horizon so that lit areas exist
         d[1] = abs(d[1]) + 0.2
+        d /= np.linalg.norm(d)
+    if front and d[2] < 0.3:   # on the camera's side of layers facing +z
+        d[2] = abs(d[2]) + 0.3
         d /= np.linalg.norm(d)
     return tuple(float(np.float32(x)) for x in d * radius)
 
@@ -578,7 +581,8 @@ def config5_scene(layers: int = 8, grid: int = 1119, width: int = 3840, height: 
     flip = fz < 0
     mesh.idx[flip] = mesh.idx[flip][:, [0, 2, 1]]
     d, n = texture_pair(tex_size, tex_size, 51)
-    return Scene(mesh, width, height, NORMALMAP, True, (0.0, 0.0, cam_z), random_light(rng), d, n,
+    # the light faces the layers too, so that the shadow pre-pass rasterises them
+    return Scene(mesh, width, height, NORMALMAP, True, (0.0, 0.0, cam_z), random_light(rng, front=True), d, n,
                  shadow_size=(shadow_size, shadow_size), name="C5")
 
 

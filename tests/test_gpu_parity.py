@@ -151,3 +151,20 @@ def test_config2_full(ctx, oracle):
     scene = sg.config2_scene()
     rep, st = check_scene(ctx, oracle, scene)
     assert st["fragments"] > 500_000
+
+
+def test_config5_reduced(ctx, oracle):
+    """Micro-triangles (1-4 px) in layer pairs with identical positions: every
+    covered pixel of the front pair is an exact depth tie, resolved by
+    submission order (LEQUAL, the later face wins), in both passes."""
+    scene = sg.config5_scene(layers=8, grid=180, width=640, height=360, shadow_size=512, tex_size=256)
+    rep, st = check_scene(ctx, oracle, scene)
+    assert st["fragments"] > scene.width * scene.height      # ties pass again
+
+
+@pytest.mark.slow
+def test_config5_full(ctx, oracle):
+    """The full micro-triangle stress frame: 20 M triangles, 3840x2160, 4096^2 shadow map."""
+    scene = sg.config5_scene()
+    rep, st = check_scene(ctx, oracle, scene)
+    assert st["faces"] == 19_998_784
