@@ -95,6 +95,8 @@ struct hana_ctx {
     uint64_t* stats_host = nullptr;    // pinned, per pass
     int stats_passes = 0;
     bool status_pending = false;
+    // device staging for hana_target_present
+    hana::rt::DevBuf present;
     // optional per-stage timing
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;

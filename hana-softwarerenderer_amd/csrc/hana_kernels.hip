@@ -769,6 +769,19 @@ __global__ void k_fill32(uint32_t* dst, uint32_t value, size_t n) {
     for (; i < n; i += stride) dst[i] = value;
 }
 
+// window_draw_buffer (SR/win32.cpp:348-370) without the window: row r of the
+// target lands on row h-1-r, R and B bytes swap (RGBA -> BGRA), alpha is
+// carried along.  One word per lane, coalesced reads and writes per row.
+__global__ void k_present(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, int w, int h) {
+    size_t n = (size_t)w * h;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        size_t r = i / (size_t)w, c = i - r * (size_t)w;
+        uint32_t v = src[i];
+        dst[((size_t)(h - 1) - r) * (size_t)w + c] = ((v & 0xFFu) << 16) | (v & 0xFF00FF00u) | ((v >> 16) & 0xFFu);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
@@ -801,6 +814,14 @@ hipError_t launch_raster(const PassDesc* d_passes, int npasses, const Workspace&
     hipLaunchKernelGGL(k_raster, dim3(blocks), dim3(BLOCK), 0, s, d_passes, npasses, ws);
     return hipGetLastError();
 }
+hipError_t launch_present(const uint32_t* src, uint32_t* dst, int w, int h, hipStream_t s) {
+    size_t n = (size_t)w * h;
+    if (n == 0) return hipSuccess;
+    int blocks = (int)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_present, dim3(blocks), dim3(256), 0, s, src, dst, w, h);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill32(uint32_t* dst, uint32_t value, size_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     int blocks = (int)std::min<size_t>((n + 255) / 256, 8192);

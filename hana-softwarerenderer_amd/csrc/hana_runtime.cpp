@@ -461,6 +461,7 @@ int hana_gpu_destroy(hana_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     free_arena(c->arena);
+    if (c->present.p) (void)hipFree(c->present.p);
     for (int i = 0; i < kRing; i++) {
         if (c->ring_host[i]) (void)hipHostFree(c->ring_host[i]);
         if (c->ring_event[i]) (void)hipEventDestroy(c->ring_event[i]);
@@ -662,6 +663,19 @@ int hana_target_read(hana_ctx* c, hana_target* t, uint8_t* rgba, float* depth) {
     size_t n = (size_t)t->w * t->h;
     if (rgba) HIP_TRY(c, hipMemcpyAsync(rgba, t->color, 4 * n, hipMemcpyDeviceToHost, c->stream));
     if (depth) HIP_TRY(c, hipMemcpyAsync(depth, t->depth, 4 * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return check_status(c);
+}
+
+int hana_target_present(hana_ctx* c, hana_target* t, uint8_t* bgra) {
+    if (!c || !t || !bgra) return fail(c, HANA_E_INVALID, "hana_target_present: null argument");
+    int r = materialize_clear(c, t);
+    if (r) return r;
+    size_t n = (size_t)t->w * t->h;
+    if ((r = ensure(c, c->present, 4 * n))) return r;
+    uint32_t* dst = c->present.as<uint32_t>();
+    HIP_TRY(c, launch_present(t->color, dst, t->w, t->h, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(bgra, dst, 4 * n, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return check_status(c);
 }

@@ -78,6 +78,16 @@ class ServerStats(C.Structure):
         return {k: (float(getattr(self, k)) if k == "busy_ms" else int(getattr(self, k))) for k, _ in self._fields_}
 
 
+class ObjMesh(C.Structure):
+    _fields_ = [("nverts", C.c_int32), ("nfaces", C.c_int32), ("pos", _FP), ("nrm", _FP), ("uv", _FP),
+                ("idx", C.POINTER(C.c_int32))]
+
+
+class ImageS(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("bytespp", C.c_int32), ("pad_", C.c_int32),
+                ("bytes", C.POINTER(C.c_uint8))]
+
+
 DECLS = [
     ("hana_gpu_create", C.c_int, [C.c_int, C.POINTER(_P)]),
     ("hana_gpu_destroy", C.c_int, [_P]),
@@ -98,6 +108,7 @@ DECLS = [
     ("hana_target_read", C.c_int, [_P, _P, C.POINTER(C.c_uint8), _FP]),
     ("hana_target_enable_winner", C.c_int, [_P, _P, C.c_int]),
     ("hana_target_read_winner", C.c_int, [_P, _P, C.POINTER(C.c_uint32)]),
+    ("hana_target_present", C.c_int, [_P, _P, C.POINTER(C.c_uint8)]),
     ("hana_draw", C.c_int, [_P, C.POINTER(DrawDesc)]),
     ("hana_render_frame", C.c_int, [_P, C.POINTER(FrameDesc)]),
     ("hana_batch_create", C.c_int, [_P, C.POINTER(_P)]),
@@ -114,6 +125,12 @@ DECLS = [
     ("hana_server_destroy", C.c_int, [_P]),
     ("hana_build_uniforms", C.c_int, [_FP, _FP, C.c_float, _FP, _FP, _FP, _FP, C.c_int32, C.c_int32, C.c_int32,
                                       C.POINTER(Uniforms)]),
+    ("hana_asset_last_error", C.c_char_p, []),
+    ("hana_obj_load", C.c_int, [C.c_char_p, C.POINTER(ObjMesh)]),
+    ("hana_obj_free", C.c_int, [C.POINTER(ObjMesh)]),
+    ("hana_tga_load", C.c_int, [C.c_char_p, C.c_int, C.POINTER(ImageS)]),
+    ("hana_tga_save", C.c_int, [C.c_char_p, C.POINTER(ImageS), C.c_int]),
+    ("hana_image_free", C.c_int, [C.POINTER(ImageS)]),
 ]
 
 _lib = None
@@ -165,6 +182,54 @@ def build_uniforms(scene, enable_shadow=None) -> Uniforms:
     if rc:
         raise HanaError(rc, "hana_build_uniforms")
     return u
+
+
+def _asset_check(lib, rc, what):
+    if rc:
+        msg = lib.hana_asset_last_error()
+        raise HanaError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+
+def load_obj(path):
+    """hana_obj_load: a Wavefront OBJ as the renderer's Model reads it, as a
+    scenegen Mesh (one vertex per distinct position/uv/fetched normal)."""
+    from .scenegen import Mesh
+    lib = load_library()
+    m = ObjMesh()
+    _asset_check(lib, lib.hana_obj_load(os.fsencode(path), C.byref(m)), f"hana_obj_load({path})")
+    try:
+        V, F = m.nverts, m.nfaces
+        pos = np.ctypeslib.as_array(m.pos, (max(V, 1) * 3,))[: 3 * V].reshape(V, 3).copy()
+        nrm = np.ctypeslib.as_array(m.nrm, (max(V, 1) * 3,))[: 3 * V].reshape(V, 3).copy()
+        uv = np.ctypeslib.as_array(m.uv, (max(V, 1) * 2,))[: 2 * V].reshape(V, 2).copy()
+        idx = np.ctypeslib.as_array(m.idx, (max(F, 1) * 3,))[: 3 * F].reshape(F, 3).copy()
+    finally:
+        lib.hana_obj_free(C.byref(m))
+    return Mesh(pos, nrm, uv, idx, fetched_normals=True)
+
+
+def load_tga(path, flip_vertically=False) -> np.ndarray:
+    """hana_tga_load: (H, W) uint8 for 1 byte per texel, else (H, W, bpp), in
+    TGAImage memory order (B, G, R, A)."""
+    lib = load_library()
+    im = ImageS()
+    _asset_check(lib, lib.hana_tga_load(os.fsencode(path), int(flip_vertically), C.byref(im)), f"hana_tga_load({path})")
+    try:
+        n = im.width * im.height * im.bytespp
+        a = np.ctypeslib.as_array(im.bytes, (n,)).copy()
+    finally:
+        shape = (im.height, im.width) if im.bytespp == 1 else (im.height, im.width, im.bytespp)
+        lib.hana_image_free(C.byref(im))
+    return a.reshape(shape)
+
+
+def save_tga(path, img: np.ndarray, rle=True) -> None:
+    """hana_tga_save: writes a TGAImage-ordered (H, W[, bpp]) uint8 array."""
+    lib = load_library()
+    a = np.ascontiguousarray(img, np.uint8)
+    bpp = 1 if a.ndim == 2 else a.shape[2]
+    im = ImageS(a.shape[1], a.shape[0], bpp, 0, a.ctypes.data_as(C.POINTER(C.c_uint8)))
+    _asset_check(lib, lib.hana_tga_save(os.fsencode(path), C.byref(im), int(rle)), f"hana_tga_save({path})")
 
 
 class Context:
@@ -271,6 +336,12 @@ class Context:
         win = np.zeros((h, w), np.uint32)
         self.check(self.lib.hana_target_read_winner(self.h, t, win.ctypes.data_as(C.POINTER(C.c_uint32))), "winner")
         return win
+
+    def present(self, t, w, h):
+        """hana_target_present: the window blit's top-down BGRA8 image."""
+        out = np.zeros((h, w, 4), np.uint8)
+        self.check(self.lib.hana_target_present(self.h, t, out.ctypes.data_as(C.POINTER(C.c_uint8))), "present")
+        return out
 
     # -- drawing --
     def draw(self, mesh, kind, u: Uniforms, target, diffuse=None, normal=None, shadow_map=None):

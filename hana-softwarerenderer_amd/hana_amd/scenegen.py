@@ -227,6 +227,10 @@ class Mesh:
     nrm: np.ndarray      # (V,3) float32
     uv: np.ndarray       # (V,2) float32
     idx: np.ndarray      # (F,3) int32
+    # normals already are what Model::normal returns at each corner (an OBJ
+    # read by hana_obj_load); otherwise they are stored vectors that every
+    # fetch renormalises (generated meshes hold fixed points, so both agree)
+    fetched_normals: bool = False
 
     @property
     def nverts(self) -> int:

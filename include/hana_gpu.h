@@ -15,6 +15,9 @@
  *   hana_target_*        RenderBuffer ctor / clears / buffers       SR/renderbuffer.h:5-21, SR/renderbuffer.cpp:3-76
  *   hana_uniforms        ShaderData + Material (no pointers)        SR/IShader.h:7-32
  *   hana_build_uniforms  DrawModel::draw uniform set-up             SR/scene.h:55-71, SR/camera.cpp:82-92
+ *   hana_target_present  window_draw_buffer (BGR swap + row flip)   SR/win32.cpp:348-370
+ *   hana_obj_load        Model::Model OBJ parser + corner fetches   SR/model.cpp:6-48,66-111
+ *   hana_tga_load/save   TGAImage::read_tga_file / write_tga_file   SR/tgaimage.cpp:40-246
  *   HANA_SHADER_*        the IShader subclasses                     SR/IShader.h:132-165
  *
  * A context owns one HIP stream and every device object created on it; calls
@@ -153,6 +156,11 @@ int hana_target_read(hana_ctx* ctx, hana_target* t, uint8_t* rgba, float* depth)
 int hana_target_enable_winner(hana_ctx* ctx, hana_target* t, int enable);
 int hana_target_read_winner(hana_ctx* ctx, hana_target* t, uint32_t* winner);
 
+/* window_draw_buffer without a window (SR/win32.cpp:348-370): the target's
+ * colour as a top-down BGRA8 image, row r of the target at row H-1-r, bytes
+ * B,G,R,A (alpha carried from the target).  bgra holds W*H*4 bytes; waits. */
+int hana_target_present(hana_ctx* ctx, hana_target* t, uint8_t* bgra);
+
 /* ---- drawing ---- */
 int hana_draw(hana_ctx* ctx, const hana_draw_desc* d);
 int hana_render_frame(hana_ctx* ctx, const hana_frame_desc* f);
@@ -222,6 +230,46 @@ int hana_server_release(hana_server* s, uint64_t id);
 int hana_server_get_stats(hana_server* s, hana_server_stats* out);
 double hana_server_now_ms(hana_server* s);
 int hana_server_destroy(hana_server* s);
+
+/* ---- on-disk formats (host only, no context) ----
+ * Errors return HANA_E_INVALID / HANA_E_NOMEM with hana_asset_last_error()
+ * describing them (per thread). */
+const char* hana_asset_last_error(void);
+
+/* A Wavefront OBJ as Model reads it (SR/model.cpp:6-48: "v", "vt", "vn" and
+ * "f v/vt/vn ..." lines, 1-based, the first three corners of a face used).
+ * Normals are the values graphics_draw_triangle's first pass over a freshly
+ * loaded Model fetches: Model::normal renormalises the stored vector at every
+ * fetch (SR/model.cpp:108-111), corners in face order.  One vertex per
+ * distinct (position, uv, fetched normal); arrays malloc'ed, freed by
+ * hana_obj_free.  Faces with fewer than three v/vt/vn corners, or indexing
+ * past a list, are rejected (the reference reads out of bounds there). */
+typedef struct hana_obj_mesh {
+    int32_t nverts, nfaces;
+    float* pos;                     /* 3*nverts */
+    float* nrm;                     /* 3*nverts */
+    float* uv;                      /* 2*nverts */
+    int32_t* idx;                   /* 3*nfaces */
+} hana_obj_mesh;
+int hana_obj_load(const char* path, hana_obj_mesh* out);
+int hana_obj_free(hana_obj_mesh* m);
+
+/* TGAImage memory layout: bytespp in {1,3,4}, B,G,R(,A) bytes, texel (x,y)
+ * at (x + y*width)*bytespp. */
+typedef struct hana_image {
+    int32_t width, height, bytespp, pad_;
+    uint8_t* bytes;
+} hana_image;
+/* TGAImage::read_tga_file (SR/tgaimage.cpp:40-143): raw (2, 3) or RLE
+ * (10, 11) data read right after the 18-byte header; flipped to top-down
+ * unless the descriptor says so, mirrored if right-to-left.  flip_vertically
+ * adds the extra flip of Model::load_texture (SR/model.cpp:80-81).  A
+ * malformed RLE stream is an error (the reference writes past its buffer). */
+int hana_tga_load(const char* path, int flip_vertically, hana_image* out);
+/* TGAImage::write_tga_file (SR/tgaimage.cpp:145-246): top-down header, raw or
+ * RLE packets as the reference forms them, TGA 2.0 footer; byte-identical. */
+int hana_tga_save(const char* path, const hana_image* img, int rle);
+int hana_image_free(hana_image* img);
 
 /* ---- host uniform math ---- */
 /* ShaderData as DrawModel::draw fills it for Camera(cam_pos, cam_target,

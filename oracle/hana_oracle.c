@@ -712,8 +712,10 @@ int ho_draw(const ho_mesh* mesh, int kind, const ho_uniforms* u, const ho_image*
             memcpy(a.obj_pos, mesh->pos + 3 * (size_t)ip, 3 * sizeof(float));
             /* Model::normal normalises the stored normal in place (SR/model.cpp:108-111) */
             float* np = mesh->normals + 3 * (size_t)in_;
-            v3 n = normalize3(mk3(np[0], np[1], np[2]));
-            np[0] = n.x; np[1] = n.y; np[2] = n.z;
+            if (!(mesh->flags & HO_MESH_FETCHED_NORMALS)) {
+                v3 n = normalize3(mk3(np[0], np[1], np[2]));
+                np[0] = n.x; np[1] = n.y; np[2] = n.z;
+            }
             memcpy(a.obj_normal, np, 3 * sizeof(float));
             memcpy(a.uv, mesh->uvs + 2 * (size_t)iu, 2 * sizeof(float));
             in[j] = vertex_impl(&c, kind, &a);

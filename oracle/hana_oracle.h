@@ -95,9 +95,13 @@ typedef struct {
 } ho_target;
 
 /* Model accessors (SR/model.cpp:66-111) over index triplets.  normals are
- * normalised in place on every fetch, as Model::normal(iface, n) does. */
+ * normalised in place on every fetch, as Model::normal(iface, n) does, unless
+ * flags has HO_MESH_FETCHED_NORMALS: then they already are the values the
+ * fetches return (as an OBJ loader that replays the fetch sequence gives
+ * them) and are used as stored. */
+#define HO_MESH_FETCHED_NORMALS 1
 typedef struct {
-    int32_t nfaces, pad_;
+    int32_t nfaces, flags;
     const float* pos;          /* 3 per vertex */
     float* normals;            /* 3 per normal, mutated */
     const float* uvs;          /* 2 per uv */

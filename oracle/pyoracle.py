@@ -52,7 +52,7 @@ class Target(C.Structure):
 
 
 class MeshS(C.Structure):
-    _fields_ = [("nfaces", C.c_int32), ("pad_", C.c_int32), ("pos", C.c_void_p), ("normals", C.c_void_p),
+    _fields_ = [("nfaces", C.c_int32), ("flags", C.c_int32), ("pos", C.c_void_p), ("normals", C.c_void_p),
                 ("uvs", C.c_void_p), ("idx_pos", C.c_void_p), ("idx_nrm", C.c_void_p), ("idx_uv", C.c_void_p)]
 
 
@@ -167,6 +167,45 @@ class CPURenderer:
             self._draw_model.argtypes = [C.c_char_p, C.c_int, C.POINTER(Uniforms), C.POINTER(Image),
                                          C.POINTER(Image), C.POINTER(Target), C.POINTER(Target)]
 
+        self._model_corners = getattr(L, "ref_model_corners", None)
+        if self._model_corners is not None:
+            self._model_corners.restype = C.c_int
+            self._model_corners.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+            self._tga_read = L.ref_tga_read
+            self._tga_read.restype = C.c_int
+            self._tga_read.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int), C.c_void_p, C.c_size_t]
+            self._tga_write = L.ref_tga_write
+            self._tga_write.restype = C.c_int
+            self._tga_write.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int]
+
+    # -- the reference's own OBJ / TGA code (reference library only) -----
+    def model_corners(self, obj_path, max_faces=1 << 24):
+        """Per-corner vert/normal/uv as graphics_draw_triangle fetches them
+        from a Model read from obj_path: (F*3, 3), (F*3, 3), (F*3, 2)."""
+        path = os.fsencode(obj_path)
+        nf = self._model_corners(path, None, None, None, 0)
+        pos = np.zeros((3 * nf, 3), np.float32)
+        nrm = np.zeros((3 * nf, 3), np.float32)
+        uv = np.zeros((3 * nf, 2), np.float32)
+        self._model_corners(path, _ptr(pos), _ptr(nrm), _ptr(uv), min(nf, max_faces))
+        return pos, nrm, uv
+
+    def tga_read(self, path, flip_vertically=False, cap=1 << 26):
+        w, h, b = C.c_int(), C.c_int(), C.c_int()
+        buf = np.zeros(cap, np.uint8)
+        ok = self._tga_read(os.fsencode(path), int(flip_vertically), C.byref(w), C.byref(h), C.byref(b), _ptr(buf), cap)
+        if not ok:
+            return None
+        n = w.value * h.value * b.value
+        shape = (h.value, w.value) if b.value == 1 else (h.value, w.value, b.value)
+        return buf[:n].reshape(shape).copy()
+
+    def tga_write(self, path, img, rle=True):
+        a = np.ascontiguousarray(img, np.uint8)
+        bpp = 1 if a.ndim == 2 else a.shape[2]
+        return bool(self._tga_write(os.fsencode(path), a.shape[1], a.shape[0], bpp, _ptr(a), int(rle)))
+
     # -- small helpers ---------------------------------------------------
     @staticmethod
     def _f(a):
@@ -273,7 +312,8 @@ class CPURenderer:
     def _mesh(self, mesh):
         keep = [np.ascontiguousarray(mesh.pos, np.float32), np.array(mesh.nrm, np.float32, copy=True),
                 np.ascontiguousarray(mesh.uv, np.float32), np.ascontiguousarray(mesh.idx, np.int32)]
-        m = MeshS(mesh.nfaces, 0, _ptr(keep[0]), _ptr(keep[1]), _ptr(keep[2]), _ptr(keep[3]), None, None)
+        flags = 1 if getattr(mesh, "fetched_normals", False) else 0     # HO_MESH_FETCHED_NORMALS
+        m = MeshS(mesh.nfaces, flags, _ptr(keep[0]), _ptr(keep[1]), _ptr(keep[2]), _ptr(keep[3]), None, None)
         return m, keep
 
     def draw(self, mesh, kind, u: Uniforms, target: HostTarget, diffuse=None, normal=None,

@@ -307,10 +307,14 @@ int ref_draw(const ho_mesh* mesh, int kind, const ho_uniforms* u, const ho_image
             shader_struct_a2v a2v;
             a2v.obj_pos = v3(mesh->pos + 3 * (size_t)ip);
             Vector3f n = v3(mesh->normals + 3 * (size_t)in_);   // Model::normal: normalise in place
-            a2v.obj_normal = n.normalize();
-            mesh->normals[3 * (size_t)in_ + 0] = n.x;
-            mesh->normals[3 * (size_t)in_ + 1] = n.y;
-            mesh->normals[3 * (size_t)in_ + 2] = n.z;
+            if (mesh->flags & HO_MESH_FETCHED_NORMALS) {
+                a2v.obj_normal = n;
+            } else {
+                a2v.obj_normal = n.normalize();
+                mesh->normals[3 * (size_t)in_ + 0] = n.x;
+                mesh->normals[3 * (size_t)in_ + 1] = n.y;
+                mesh->normals[3 * (size_t)in_ + 2] = n.z;
+            }
             a2v.uv = Vector2f(mesh->uvs[2 * (size_t)iu], mesh->uvs[2 * (size_t)iu + 1]);
             v2fs[j] = dd.shader->vertex(&a2v);
         }
@@ -357,6 +361,49 @@ int ref_draw_model(const char* obj_path, int kind, const ho_uniforms* u, const h
     delete rb;
     delete shader;
     return model.nfaces();
+}
+
+// What graphics_draw_triangle's face loop fetches from a Model read from an
+// OBJ file (SR/graphics.cpp:380-386): per corner, in order, vert / normal / uv.
+// Returns the face count; fills at most max_faces faces.
+int ref_model_corners(const char* obj_path, float* pos, float* nrm, float* uv, int max_faces) {
+    Model model(obj_path);
+    int nf = model.nfaces();
+    for (int i = 0; i < nf && i < max_faces; i++)
+        for (int j = 0; j < 3; j++) {
+            size_t c = 3 * (size_t)i + j;
+            Vector3f p = model.vert(i, j);
+            Vector3f n = model.normal(i, j);
+            Vector2f t = model.uv(i, j);
+            for (int k = 0; k < 3; k++) {
+                pos[3 * c + k] = p[k];
+                nrm[3 * c + k] = n[k];
+            }
+            uv[2 * c] = t[0];
+            uv[2 * c + 1] = t[1];
+        }
+    return nf;
+}
+
+// TGAImage::read_tga_file (+ Model::load_texture's flip when flip_v).  Returns
+// 1 on success and copies at most cap bytes.
+int ref_tga_read(const char* path, int flip_v, int* w, int* h, int* bpp, unsigned char* out, size_t cap) {
+    TGAImage img;
+    if (!img.read_tga_file(path)) return 0;
+    if (flip_v) img.flip_vertically();
+    *w = img.get_width();
+    *h = img.get_height();
+    *bpp = img.get_bytespp();
+    size_t n = (size_t)*w * *h * *bpp;
+    memcpy(out, img.buffer(), n < cap ? n : cap);
+    return 1;
+}
+
+// TGAImage::write_tga_file of an image with the given bytes.
+int ref_tga_write(const char* path, int w, int h, int bpp, const unsigned char* bytes, int rle) {
+    TGAImage img(w, h, bpp);
+    memcpy(img.buffer(), bytes, (size_t)w * h * bpp);
+    return img.write_tga_file(path, rle != 0) ? 1 : 0;
 }
 
 void ref_target_clear(ho_target* t, const float* c, float depth) {
