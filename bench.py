@@ -273,8 +273,7 @@ def alg_bytes(scene):
     W, H = scene.width, scene.height
     S = scene.shadow_wh[0] * scene.shadow_wh[1] if scene.shadow else 0
     tex = scene.diffuse.shape[0] * scene.diffuse.shape[1] * 4 + scene.normal.shape[0] * scene.normal.shape[1] * 4
-    b = {"vertex": passes * V * 32, "setup_count": passes * T * 12, "setup_scatter": passes * T * 12,
-         "clip_count": 0, "clip_scatter": 0, "scan": 0,
+    b = {"vertex": passes * V * 32, "setup": passes * T * 12, "bin": 0, "clip": 0, "scan": 0, "unused": 0,
          "raster_g0": S * 8, "raster_g1": W * H * 8 + S * 4 + tex}
     if not scene.shadow:
         b["raster_g0"] = W * H * 8 + tex

@@ -53,9 +53,9 @@ struct PassSpec {
 
 // Grow-only device workspace for launch sequences.
 struct Arena {
-    DevBuf descs, unis, vr, vs, vis, pof, tile_count, tile_offset, tile_cursor, bins;
+    DevBuf descs, unis, vr, vs, code, pof, tile_count, tile_offset, tile_cursor, bins, tris, scan_part;
     DevBuf pool_vr, pool_vs, pool_face, pool_pass, pool_n, clip_queue, status, stats;
-    uint32_t bin_cap = 0, poly_cap = 0, clipq_cap = 0;
+    uint32_t bin_cap = 0, poly_cap = 0, clipq_cap = 0, tris_cap = 0;
 };
 
 struct Plan {
@@ -67,7 +67,7 @@ struct Plan {
     int vblocks = 0, fblocks = 0;
     int ngroups = 0;
     int group_first[2] = {0, 0}, group_count[2] = {0, 0}, group_blocks[2] = {0, 0};
-    size_t total_v = 0, total_vs = 0, total_vis = 0, total_f = 0;
+    size_t total_v = 0, total_vs = 0, total_f = 0;
     int total_tiles = 0;
     Workspace ws{};
 };

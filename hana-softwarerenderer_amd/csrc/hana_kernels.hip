@@ -3,16 +3,18 @@
 // One launch sequence renders any number of passes (a pass is one
 // graphics_draw_triangle call, SR/graphics.cpp:378-407):
 //
-//   k_vertex   VPT vertices per lane: vertex program, clip-space trivial-accept
-//              bit, perspective divide and 1/w          (SR/IShader.cpp, SR/graphics.cpp:132-134, 316-337)
-//   k_setup<0> FPT faces per lane: back-face cull, viewport, bbox, degenerate
-//              test of trivially accepted faces; counts (triangle, tile)
-//              pairs; queues the other faces for clipping
-//   k_clip     clips the queued faces (SR/graphics.cpp:136-161) into polygons
-//              and counts their fan triangles
-//   k_scan     exclusive scan of the per-tile counts
-//   k_setup<1> same set-up; scatters triangle ids into the per-tile bins
-//   k_clip_scatter  scatters the clipped fan triangles
+//   k_vertex   VPT vertices per lane: vertex program, clip-space outcode
+//              (is_vertex_visible and the 7 plane tests), perspective divide
+//              and 1/w                                (SR/IShader.cpp, SR/graphics.cpp:29-49, 132-134, 316-337)
+//   k_setup    FPT faces per lane: back-face cull, viewport, bbox, degenerate
+//              test of trivially accepted faces; counts their (triangle,
+//              tile) pairs and lists them for binning; drops faces that the
+//              first plane they cross removes whole; queues the rest for
+//              clipping
+//   k_clip     clips the queued faces (SR/graphics.cpp:136-161) into polygons,
+//              counts and lists their fan triangles
+//   k_scan_*   exclusive scan of the per-tile counts (reduce, then apply)
+//   k_bin      scatters every listed triangle into its tiles' bins
 //   k_raster   one workgroup per 16x16 tile: exact per-pixel coverage and
 //              depth (SR/graphics.cpp:222-233, 349-359) resolved in LDS with
 //              64-bit (depth, submission order) keys, then one fragment
@@ -54,7 +56,6 @@ __device__ __forceinline__ int find_pass(const PassDesc* P, int n, int block) {
     return lo;
 }
 
-__device__ __forceinline__ bool vis_bit(const uint64_t* vis, int i) { return (vis[i >> 6] >> (i & 63)) & 1ull; }
 
 // Triangle set-up in screen space (SR/graphics.cpp:316-347, SR/maths.cpp:20-25).
 struct TriS {
@@ -125,6 +126,33 @@ __device__ __forceinline__ bool tri_cover(const TriS& t, int px, int py, float& 
     return true;
 }
 
+// A division-free test that rejects only pixels tri_cover rejects too (it
+// never rejects a covered one), so that the three divisions run only where a
+// fragment can result.  With |u.z| <= 2^60:
+//   w.y = fl(u.y/u.z) < 0 for sure if u.y and u.z differ in sign and
+//     |u.y| >= 2^-60 (the quotient is then at least 2^-120 in magnitude and
+//     cannot round to -0); w.z likewise with u.x;
+//   w.x = fl(1 - fl(s/u.z)) < 0, s = fl(u.x + u.y), for sure if
+//     s/u.z > 1 + 2^-17: tested as s > fl(u.z*(1+2^-16)) for u.z > 0 (or <
+//     for u.z < 0); fl(u.z*(1+2^-16)) errs by at most 2^-24 relative, so the
+//     quotient exceeds 1 + 2^-17 and rounds above 1.
+// NaN operands fail every comparison and are left to tri_cover.
+__device__ __forceinline__ bool maybe_covered(const TriS& t, int px, int py) {
+    float Px = (float)px, Py = (float)py;
+    float s0z = t.ax - Px;
+    float s1z = t.ay - Py;
+    float ux = t.s0y * s1z - s0z * t.s1y;
+    float uy = s0z * t.s1x - t.s0x * s1z;
+    if (!(fabsf(t.uz) <= 0x1p60f)) return true;
+    uint32_t sz = __float_as_uint(t.uz) & 0x80000000u;
+    if (fabsf(uy) >= 0x1p-60f && (__float_as_uint(uy) & 0x80000000u) != sz) return false;
+    if (fabsf(ux) >= 0x1p-60f && (__float_as_uint(ux) & 0x80000000u) != sz) return false;
+    float s = ux + uy;
+    float lim = t.uz * 1.0000152587890625f;   // 1 + 2^-16
+    if (sz ? (s < lim) : (s > lim)) return false;
+    return true;
+}
+
 // order-preserving 32-bit image of a depth (-0 folded onto +0)
 __device__ __forceinline__ uint32_t depth_bits(float z) {
     uint32_t u = __float_as_uint(z);
@@ -138,6 +166,22 @@ __device__ __forceinline__ uint64_t init_key(float d) {
 
 __device__ __forceinline__ void stat_add(const PassDesc& P, int which, uint64_t v) {
     if (P.stats && v) atomicAdd((unsigned long long*)&P.stats[which], (unsigned long long)v);
+}
+
+// clip planes in clip_triangle's order (SR/graphics.cpp:136-161) and their
+// inside tests (is_inside_plane, SR/graphics.cpp:29-49)
+enum { PL_W, PL_PX, PL_NX, PL_PY, PL_NY, PL_PZ, PL_NZ };
+
+__device__ __forceinline__ bool inside_plane(const float* c, int plane) {
+    switch (plane) {
+    case PL_W: return c[3] >= 1e-5f;
+    case PL_PX: return c[0] <= +c[3];
+    case PL_NX: return c[0] >= -c[3];
+    case PL_PY: return c[1] <= +c[3];
+    case PL_NY: return c[1] >= -c[3];
+    case PL_PZ: return c[2] <= +c[3];
+    default: return c[2] >= -c[3];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -173,12 +217,16 @@ __global__ __launch_bounds__(BLOCK) void k_vertex(const PassDesc* passes, int np
 #pragma unroll
     for (int k = 0; k < VPT; k++) {
         int v = base + k * BLOCK + threadIdx.x;
-        bool vis = false;
         if (v < P.nverts) {
             float clip[4], vary[8];
             vertex_program(P.kind, u, pos[k], nrm[k], tu[k], tv[k], clip, vary);
-            // is_vertex_visible (SR/graphics.cpp:132-134)
-            vis = fabsf(clip[0]) <= clip[3] && fabsf(clip[1]) <= clip[3] && fabsf(clip[2]) <= clip[3];
+            // is_vertex_visible (SR/graphics.cpp:132-134), then the plane tests
+            uint32_t code =
+                (fabsf(clip[0]) <= clip[3] && fabsf(clip[1]) <= clip[3] && fabsf(clip[2]) <= clip[3]) ? CODE_VISIBLE : 0u;
+#pragma unroll
+            for (int pl = 0; pl < 7; pl++)
+                if (!inside_plane(clip, pl)) code |= 1u << pl;
+            P.code[v] = (uint8_t)code;
             // clip_pos / clip_pos[3] and 1 / clip_pos[3] (SR/graphics.cpp:317, 336)
             float4 r;
             r.w = 1 / clip[3];
@@ -196,9 +244,6 @@ __global__ __launch_bounds__(BLOCK) void k_vertex(const PassDesc* passes, int np
                 P.vs[v] = vary[0];
             }
         }
-        uint64_t bits = __ballot(vis);
-        int wave_first = base + k * BLOCK + (threadIdx.x & ~63);
-        if ((threadIdx.x & 63) == 0 && wave_first < P.nverts) P.vis[wave_first >> 6] = bits;
     }
 }
 
@@ -206,19 +251,6 @@ __global__ __launch_bounds__(BLOCK) void k_vertex(const PassDesc* passes, int np
 // clipping (SR/graphics.cpp:7-161), for faces failing the trivial accept
 // ---------------------------------------------------------------------------
 
-enum { PL_W, PL_PX, PL_NX, PL_PY, PL_NY, PL_PZ, PL_NZ };
-
-__device__ __forceinline__ bool inside_plane(const float* c, int plane) {
-    switch (plane) {
-    case PL_W: return c[3] >= 1e-5f;
-    case PL_PX: return c[0] <= +c[3];
-    case PL_NX: return c[0] >= -c[3];
-    case PL_PY: return c[1] <= +c[3];
-    case PL_NY: return c[1] >= -c[3];
-    case PL_PZ: return c[2] <= +c[3];
-    default: return c[2] >= -c[3];
-    }
-}
 __device__ __forceinline__ float intersect_ratio(const float* p, const float* c, int plane) {
     switch (plane) {
     case PL_W: return (p[3] - 1e-5f) / (p[3] - c[3]);
@@ -340,7 +372,8 @@ __device__ __noinline__ int clip_face(const PassDesc& P, const Workspace& ws, in
 }
 
 // ---------------------------------------------------------------------------
-// binning: count (SCATTER = 0) or scatter (SCATTER = 1) (triangle, tile) pairs
+// binning: count the (triangle, tile) pairs and list the triangles; after the
+// scan, k_bin scatters the listed triangles into the bins
 // ---------------------------------------------------------------------------
 
 // Reserves one slot in counters[t] for every lane with `want`, with one atomic
@@ -368,44 +401,59 @@ __device__ __forceinline__ uint32_t wave_reserve(uint32_t* counters, int t, bool
     return mine;
 }
 
-// Emits every (triangle, tile) pair of one lane's triangle; wave-uniform.
-template <int SCATTER>
-__device__ __forceinline__ void emit_tiles(const PassDesc& P, const Workspace& ws, bool ok, const TriS& t,
-                                           uint32_t entry) {
-    int tx0 = 0, ty0 = 0, nx = 0, n = 0;
+// Counts every (triangle, tile) pair of one lane's triangle and lists the
+// triangle (bin entry, pass, first and last tile) for k_bin; wave-uniform.
+__device__ __forceinline__ void count_tiles(const PassDesc& P, const Workspace& ws, int pass, bool ok, const TriS& t,
+                                            uint32_t entry) {
+    int tx0 = 0, ty0 = 0, tx1 = 0, ty1 = 0, nx = 0, n = 0;
     if (ok) {
         tx0 = t.x0 >> TILE_SHIFT;
         ty0 = t.y0 >> TILE_SHIFT;
-        nx = (t.x1 >> TILE_SHIFT) - tx0 + 1;
-        n = nx * ((t.y1 >> TILE_SHIFT) - ty0 + 1);
+        tx1 = t.x1 >> TILE_SHIFT;
+        ty1 = t.y1 >> TILE_SHIFT;
+        nx = tx1 - tx0 + 1;
+        n = nx * (ty1 - ty0 + 1);
+    }
+    uint32_t slot = wave_reserve(&ws.status[STATUS_TRIS], 0, ok);
+    if (ok) {
+        if (slot < ws.tris_cap)
+            ws.tris[slot] = make_uint4(entry, (uint32_t)pass, (uint32_t)tx0 | ((uint32_t)ty0 << 16),
+                                       (uint32_t)tx1 | ((uint32_t)ty1 << 16));
+        else
+            atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_TRIS_OVERFLOW);
     }
     for (int k = 0; __ballot(k < n); k++) {
         bool want = k < n;
         int gt = want ? P.tile0 + (ty0 + k / nx) * P.tiles_x + tx0 + k % nx : 0;
-        if (!SCATTER) {
-            wave_reserve(ws.tile_count, gt, want);
-        } else {
-            uint32_t pos = wave_reserve(ws.tile_cursor, gt, want);
-            if (want && pos < ws.bin_cap) ws.bins[pos] = entry;
-        }
+        wave_reserve(ws.tile_count, gt, want);
     }
-    if (!SCATTER && P.stats) {
-        uint64_t tot = 0;
+    if (P.stats) {
         for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off);
-        tot = (uint64_t)n;
-        if (__lane_id() == 0) stat_add(P, ST_ENTRIES, tot);
+        if (__lane_id() == 0) stat_add(P, ST_ENTRIES, (uint64_t)n);
     }
 }
 
+// Faces that fail the trivial accept, from the outcodes of their vertices:
+// clip_triangle passes a polygon whose vertices are all inside a plane through
+// that plane unchanged, and returns no vertex once a plane has all of them
+// outside (SR/graphics.cpp:73-110, 150-152).  So a face dies whole if, at the
+// first plane some vertex is outside of, every vertex is outside.
+__device__ __forceinline__ bool dies_at_first_crossed_plane(uint32_t c0, uint32_t c1, uint32_t c2) {
+    uint32_t some_out = (c0 | c1 | c2) & CODE_PLANES;
+    if (!some_out) return false;
+    uint32_t first = some_out & (0u - some_out);
+    return (c0 & c1 & c2 & first) != 0u;
+}
+
 // FPT faces per lane.  Trivially accepted faces are set up from the shared
-// post-transform vertices; the others are queued for k_clip.
-template <int SCATTER>
+// post-transform vertices; faces that clipping would remove are dropped; the
+// others are queued for k_clip.
 __global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* passes, int npasses, Workspace ws) {
     int pass = find_pass<1>(passes, npasses, blockIdx.x);
     const PassDesc& P = passes[pass];
     int base = (blockIdx.x - P.fblock0) * BLOCK * FPT;
     int ids[FPT][3];
-    bool valid[FPT], trivial[FPT];
+    bool valid[FPT];
 #pragma unroll
     for (int k = 0; k < FPT; k++) {
         int f = base + k * BLOCK + threadIdx.x;
@@ -417,10 +465,15 @@ __global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* passes, int npa
             ids[k][2] = P.idx[3 * f + 2];
         }
     }
+    uint32_t codes[FPT][3];
+#pragma unroll
+    for (int k = 0; k < FPT; k++)
+        for (int j = 0; j < 3; j++) codes[k][j] = valid[k] ? P.code[ids[k][j]] : 0u;
+    bool trivial[FPT];
     float4 va[FPT], vb[FPT], vc[FPT];
 #pragma unroll
     for (int k = 0; k < FPT; k++) {
-        trivial[k] = valid[k] && vis_bit(P.vis, ids[k][0]) && vis_bit(P.vis, ids[k][1]) && vis_bit(P.vis, ids[k][2]);
+        trivial[k] = valid[k] && (codes[k][0] & codes[k][1] & codes[k][2] & CODE_VISIBLE);
         if (trivial[k]) {
             va[k] = P.vr[ids[k][0]];
             vb[k] = P.vr[ids[k][1]];
@@ -430,31 +483,32 @@ __global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* passes, int npa
 #pragma unroll
     for (int k = 0; k < FPT; k++) {
         int f = base + k * BLOCK + threadIdx.x;
-        if (!SCATTER) {
-            bool clipq = valid[k] && !trivial[k];
-            uint32_t q = wave_reserve(&ws.status[STATUS_CLIPQ], 0, clipq);
-            if (clipq && q < ws.clipq_cap) ws.clip_queue[q] = ((uint64_t)(uint32_t)pass << 32) | (uint32_t)f;
-            if (P.stats) {
-                uint64_t nv = __popcll(__ballot(valid[k])), nc = __popcll(__ballot(clipq));
-                if (__lane_id() == 0) {
-                    stat_add(P, ST_FACES, nv);
-                    stat_add(P, ST_CLIPPED, nc);
-                }
+        bool clipped = valid[k] && !trivial[k];
+        bool clipq = clipped && !dies_at_first_crossed_plane(codes[k][0], codes[k][1], codes[k][2]);
+        if (clipped && !clipq) P.poly_of_face[f] = -1;
+        uint32_t q = wave_reserve(&ws.status[STATUS_CLIPQ], 0, clipq);
+        if (clipq) {
+            if (q < ws.clipq_cap) ws.clip_queue[q] = ((uint64_t)(uint32_t)pass << 32) | (uint32_t)f;
+            else atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_CLIPQ_OVERFLOW);
+        }
+        if (P.stats) {
+            uint64_t nv = __popcll(__ballot(valid[k])), nc = __popcll(__ballot(clipped));
+            if (__lane_id() == 0) {
+                stat_add(P, ST_FACES, nv);
+                stat_add(P, ST_CLIPPED, nc);
             }
         }
         TriS t;
         bool ok = trivial[k] && tri_setup(va[k], vb[k], vc[k], P.width, P.height, t);
-        emit_tiles<SCATTER>(P, ws, ok, t, (uint32_t)f);
+        count_tiles(P, ws, pass, ok, t, (uint32_t)f);
     }
 }
 
-// Clips the queued faces (count phase): one wave per workgroup, grid-stride
-// over the queue, polygon buffers in LDS.
+// Clips the queued faces: one wave per workgroup, grid-stride over the queue,
+// polygon buffers in LDS; counts and lists the fan triangles (0, j+1, j+2).
 __global__ __launch_bounds__(CLIP_LANES) void k_clip(const PassDesc* passes, Workspace ws) {
     __shared__ float lds[2 * POLY_MAX * CLIP_NF * CLIP_LANES];
     uint32_t count = min(ws.status[STATUS_CLIPQ], ws.clipq_cap);
-    if (ws.status[STATUS_CLIPQ] > ws.clipq_cap && blockIdx.x == 0 && threadIdx.x == 0)
-        atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_CLIPQ_OVERFLOW);
     for (uint32_t base = blockIdx.x * CLIP_LANES; base < count; base += gridDim.x * CLIP_LANES) {
         uint32_t q = base + threadIdx.x;
         bool have = q < count;
@@ -468,7 +522,6 @@ __global__ __launch_bounds__(CLIP_LANES) void k_clip(const PassDesc* passes, Wor
             if (pid >= 0) stat_add(passes[pass], ST_POLYS, 1);
         }
         int n = pid >= 0 ? ws.pool_n[pid] : 0;
-        // fan triangles (0, j+1, j+2), counted per j across the wave
         for (int j = 0; __ballot(j < n - 2); j++) {
             TriS t;
             bool ok = false;
@@ -476,61 +529,104 @@ __global__ __launch_bounds__(CLIP_LANES) void k_clip(const PassDesc* passes, Wor
                 const float4* pv = ws.pool_vr + (size_t)pid * POLY_MAX;
                 ok = tri_setup(pv[0], pv[j + 1], pv[j + 2], passes[pass].width, passes[pass].height, t);
             }
-            emit_tiles<0>(passes[pass], ws, ok, t, 0u);
+            count_tiles(passes[pass], ws, pass, ok, t, CLIPPED_BIT | ((uint32_t)max(pid, 0) << 3) | (uint32_t)j);
         }
     }
 }
 
-// Scatters the fan triangles of every clipped polygon: grid-stride over the pool.
-__global__ __launch_bounds__(BLOCK) void k_clip_scatter(const PassDesc* passes, Workspace ws) {
-    uint32_t count = min(ws.status[STATUS_POLYS], ws.poly_cap);
+// Scatters every listed triangle into the bins of its tiles; grid-stride.
+__global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* passes, Workspace ws) {
+    uint32_t count = min(ws.status[STATUS_TRIS], ws.tris_cap);
     for (uint32_t base = blockIdx.x * BLOCK; base < count; base += gridDim.x * BLOCK) {
-        uint32_t pid = base + threadIdx.x;
-        bool have = pid < count;
-        int pass = have ? ws.pool_pass[pid] : 0;
-        int n = have ? ws.pool_n[pid] : 0;
-        for (int j = 0; __ballot(j < n - 2); j++) {
-            TriS t;
-            bool ok = false;
-            if (j < n - 2) {
-                const float4* pv = ws.pool_vr + (size_t)pid * POLY_MAX;
-                ok = tri_setup(pv[0], pv[j + 1], pv[j + 2], passes[pass].width, passes[pass].height, t);
-            }
-            emit_tiles<1>(passes[pass], ws, ok, t, CLIPPED_BIT | (pid << 3) | (uint32_t)j);
+        uint32_t i = base + threadIdx.x;
+        int tx0 = 0, ty0 = 0, nx = 0, n = 0, tile0 = 0, tiles_x = 0;
+        uint32_t entry = 0;
+        if (i < count) {
+            uint4 r = ws.tris[i];
+            const PassDesc& P = passes[r.y];
+            entry = r.x;
+            tx0 = (int)(r.z & 0xFFFFu);
+            ty0 = (int)(r.z >> 16);
+            nx = (int)(r.w & 0xFFFFu) - tx0 + 1;
+            n = nx * ((int)(r.w >> 16) - ty0 + 1);
+            tile0 = P.tile0;
+            tiles_x = P.tiles_x;
+        }
+        for (int k = 0; __ballot(k < n); k++) {
+            bool want = k < n;
+            int gt = want ? tile0 + (ty0 + k / nx) * tiles_x + tx0 + k % nx : 0;
+            uint32_t pos = wave_reserve(ws.tile_cursor, gt, want);
+            if (want && pos < ws.bin_cap) ws.bins[pos] = entry;
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// k_scan: exclusive scan of the tile counts (one workgroup)
+// k_scan_reduce / k_scan_apply: exclusive scan of the tile counts into
+// tile_offset (and tile_cursor, the scatter cursors).  Each workgroup owns
+// SCAN_CHUNK counters; the apply kernel adds the sums of the chunks before.
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(1024) void k_scan(Workspace ws) {
-    __shared__ uint32_t sh[1024];
-    int n = ws.ntiles, tid = threadIdx.x;
-    int per = (n + 1023) / 1024;
-    int b = tid * per, e = min(b + per, n);
-    uint32_t s = 0;
-    for (int i = b; i < e; i++) s += ws.tile_count[i];
-    sh[tid] = s;
+__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* sh) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    int w = threadIdx.x >> 6;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint32_t v = tid >= off ? sh[tid - off] : 0u;
-        __syncthreads();
-        sh[tid] += v;
-        __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    uint32_t t = 0;
+    for (int i = 0; i < SCAN_THREADS / 64; i++) t += sh[i];
+    return t;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(Workspace ws) {
+    __shared__ uint32_t sh[SCAN_THREADS / 64];
+    int n = ws.ntiles;
+    int b = blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_PER_THREAD;
+    uint32_t s = 0;
+    for (int i = 0; i < SCAN_PER_THREAD; i++)
+        if (b + i < n) s += ws.tile_count[b + i];
+    uint32_t t = block_sum(s, sh);
+    if (threadIdx.x == 0) ws.scan_part[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(Workspace ws) {
+    __shared__ uint32_t sh[SCAN_THREADS / 64];
+    __shared__ uint32_t wsum[SCAN_THREADS / 64];
+    int n = ws.ntiles;
+    int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    // sum of the chunks before this one
+    uint32_t before = 0;
+    for (int i = tid; i < (int)blockIdx.x; i += SCAN_THREADS) before += ws.scan_part[i];
+    before = block_sum(before, sh);
+    // this thread's counters, then an exclusive scan of the thread sums
+    int b = blockIdx.x * SCAN_CHUNK + tid * SCAN_PER_THREAD;
+    uint32_t v[SCAN_PER_THREAD];
+    uint32_t s = 0;
+    for (int i = 0; i < SCAN_PER_THREAD; i++) {
+        v[i] = (b + i < n) ? ws.tile_count[b + i] : 0u;
+        s += v[i];
     }
-    uint32_t run = sh[tid] - s;
-    for (int i = b; i < e; i++) {
-        ws.tile_offset[i] = run;
-        ws.tile_cursor[i] = run;
-        run += ws.tile_count[i];
+    uint32_t incl = s;
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t o = __shfl_up(incl, d);
+        if (lane >= d) incl += o;
     }
-    if (tid == 1023) {
-        uint32_t total = sh[1023];
-        ws.tile_offset[n] = total;
-        ws.status[STATUS_BINS_NEEDED] = total;
-        if (total > ws.bin_cap) atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_BIN_OVERFLOW);
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    uint32_t wbefore = 0;
+    for (int i = 0; i < w; i++) wbefore += wsum[i];
+    uint32_t run = before + wbefore + incl - s;
+    for (int i = 0; i < SCAN_PER_THREAD; i++)
+        if (b + i < n) {
+            ws.tile_offset[b + i] = run;
+            ws.tile_cursor[b + i] = run;
+            run += v[i];
+        }
+    if (blockIdx.x == gridDim.x - 1 && tid == SCAN_THREADS - 1) {
+        // `run` of the last thread is the grand total
+        ws.tile_offset[n] = run;
+        ws.status[STATUS_BINS_NEEDED] = run;
+        if (run > ws.bin_cap) atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_BIN_OVERFLOW);
     }
 }
 
@@ -538,15 +634,15 @@ __global__ __launch_bounds__(1024) void k_scan(Workspace ws) {
 // k_raster: per-tile visibility in LDS, then one fragment program per pixel
 // ---------------------------------------------------------------------------
 
-constexpr int SMALL_PIX = 16;   // triangles with at most this many candidate pixels: one lane each
+constexpr int SMALL_PIX = HANA_SMALL_PIX;   // triangles with at most this many candidate pixels: one lane each
 
 struct TileLds {
     uint64_t key[TILE_PIX];
     uint32_t nan_order[TILE_PIX];   // 1 + order of the last NaN-depth fragment
-    float ax[BLOCK], ay[BLOCK], s0x[BLOCK], s0y[BLOCK], s1x[BLOCK], s1y[BLOCK], uz[BLOCK];
-    float z0[BLOCK], z1[BLOCK], z2[BLOCK];
-    uint32_t lo[BLOCK], box[BLOCK];
-    int big[BLOCK];
+    float ax[RBLOCK], ay[RBLOCK], s0x[RBLOCK], s0y[RBLOCK], s1x[RBLOCK], s1y[RBLOCK], uz[RBLOCK];
+    float z0[RBLOCK], z1[RBLOCK], z2[RBLOCK];
+    uint32_t lo[RBLOCK], box[RBLOCK];
+    int big[RBLOCK];
     int nbig, head, nan_seen;
 };
 
@@ -593,7 +689,7 @@ __device__ __forceinline__ void frag_resolve(TileLds& L, const TriS& t, uint32_t
 __device__ void sweep(TileLds& L, const PassDesc& P, const Workspace& ws, uint32_t beg, uint32_t cnt, int xb, int yb,
                       bool second) {
     int tid = threadIdx.x, lane = tid & 63;
-    for (uint32_t base = 0; base < cnt; base += BLOCK) {
+    for (uint32_t base = 0; base < cnt; base += RBLOCK) {
         bool ok = false;
         TriS t;
         uint32_t order = 0;
@@ -606,10 +702,18 @@ __device__ void sweep(TileLds& L, const PassDesc& P, const Workspace& ws, uint32
         }
         uint32_t lo = 0xFFFFFFFEu - order;
         if (ok) {
-            int npix = (bx1 - bx0 + 1) * (by1 - by0 + 1);
+            int wdt = bx1 - bx0 + 1;
+            int npix = wdt * (by1 - by0 + 1);
             if (npix <= SMALL_PIX) {
-                for (int ly = by0; ly <= by1; ly++)
-                    for (int lx = bx0; lx <= bx1; lx++) frag_resolve(L, t, lo, xb, yb, lx, ly, second);
+                // cheap pass over the candidates, then the exact test on the survivors
+                uint32_t mask = 0;
+                for (int k = 0; k < npix; k++)
+                    if (maybe_covered(t, xb + bx0 + k % wdt, yb + by0 + k / wdt)) mask |= 1u << k;
+                while (mask) {
+                    int k = __builtin_ctz(mask);
+                    mask &= mask - 1u;
+                    frag_resolve(L, t, lo, xb, yb, bx0 + k % wdt, by0 + k / wdt, second);
+                }
             } else {
                 L.ax[tid] = t.ax; L.ay[tid] = t.ay; L.s0x[tid] = t.s0x; L.s0y[tid] = t.s0y;
                 L.s1x[tid] = t.s1x; L.s1y[tid] = t.s1y; L.uz[tid] = t.uz;
@@ -637,7 +741,7 @@ __device__ void sweep(TileLds& L, const PassDesc& P, const Workspace& ws, uint32
             int npix = wdt * (y1 - y0 + 1);
             for (int k = lane; k < npix; k += 64) {
                 int ly = y0 + k / wdt, lx = x0 + k % wdt;
-                frag_resolve(L, bt, blo, xb, yb, lx, ly, second);
+                if (maybe_covered(bt, xb + lx, yb + ly)) frag_resolve(L, bt, blo, xb, yb, lx, ly, second);
             }
         }
         __syncthreads();
@@ -705,7 +809,7 @@ __device__ __forceinline__ int xcd_logical_block(int b, int nb) {
     return xcd * per + min(xcd, rem) + slot;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_raster(const PassDesc* passes, int npasses, Workspace ws) {
+__global__ __launch_bounds__(RBLOCK) void k_raster(const PassDesc* passes, int npasses, Workspace ws) {
     __shared__ TileLds L;
     int blk = xcd_logical_block(blockIdx.x, gridDim.x);
     const PassDesc& P = passes[find_pass<2>(passes, npasses, blk)];
@@ -716,7 +820,7 @@ __global__ __launch_bounds__(BLOCK) void k_raster(const PassDesc* passes, int np
     int tid = threadIdx.x;
     int W = P.width, H = P.height;
 
-    for (int i = tid; i < TILE_PIX; i += BLOCK) {
+    for (int i = tid; i < TILE_PIX; i += RBLOCK) {
         int px = xb + (i & (TILE - 1)), py = yb + (i >> TILE_SHIFT);
         uint64_t k = ~0ull;
         if (px < W && py < H) k = init_key(P.clear ? P.clear_depth : P.depth[(size_t)py * W + px]);
@@ -734,7 +838,7 @@ __global__ __launch_bounds__(BLOCK) void k_raster(const PassDesc* passes, int np
 
     if (L.nan_seen) {
         // pixels that saw a NaN-depth fragment restart after the last one
-        for (int i = tid; i < TILE_PIX; i += BLOCK) {
+        for (int i = tid; i < TILE_PIX; i += RBLOCK) {
             uint32_t no = L.nan_order[i];
             if (no) L.key[i] = (0xFFFFFFFFull << 32) | (uint64_t)(0xFFFFFFFEu - (no - 1u));
         }
@@ -743,7 +847,7 @@ __global__ __launch_bounds__(BLOCK) void k_raster(const PassDesc* passes, int np
     }
 
     uint32_t written = 0;
-    for (int i = tid; i < TILE_PIX; i += BLOCK) {
+    for (int i = tid; i < TILE_PIX; i += RBLOCK) {
         int px = xb + (i & (TILE - 1)), py = yb + (i >> TILE_SHIFT);
         if (px >= W || py >= H) continue;
         uint32_t lo = (uint32_t)L.key[i];
@@ -791,27 +895,33 @@ hipError_t launch_vertex(const PassDesc* d_passes, int npasses, const Workspace&
     hipLaunchKernelGGL(k_vertex, dim3(blocks), dim3(BLOCK), 0, s, d_passes, npasses, ws);
     return hipGetLastError();
 }
-hipError_t launch_setup(const PassDesc* d_passes, int npasses, const Workspace& ws, int blocks, int scatter,
-                        hipStream_t s) {
+hipError_t launch_setup(const PassDesc* d_passes, int npasses, const Workspace& ws, int blocks, hipStream_t s) {
     if (blocks <= 0) return hipSuccess;
-    if (scatter) hipLaunchKernelGGL(k_setup<1>, dim3(blocks), dim3(BLOCK), 0, s, d_passes, npasses, ws);
-    else hipLaunchKernelGGL(k_setup<0>, dim3(blocks), dim3(BLOCK), 0, s, d_passes, npasses, ws);
+    hipLaunchKernelGGL(k_setup, dim3(blocks), dim3(BLOCK), 0, s, d_passes, npasses, ws);
     return hipGetLastError();
 }
-hipError_t launch_clip(const PassDesc* d_passes, const Workspace& ws, int scatter, hipStream_t s) {
-    // grid-stride kernels: the queue length is only known on the device
-    const int blocks = 512;
-    if (scatter) hipLaunchKernelGGL(k_clip_scatter, dim3(blocks), dim3(BLOCK), 0, s, d_passes, ws);
-    else hipLaunchKernelGGL(k_clip, dim3(4 * blocks), dim3(CLIP_LANES), 0, s, d_passes, ws);
+hipError_t launch_clip(const PassDesc* d_passes, const Workspace& ws, hipStream_t s) {
+    // grid-stride: the queue length is only known on the device
+    hipLaunchKernelGGL(k_clip, dim3(2048), dim3(CLIP_LANES), 0, s, d_passes, ws);
     return hipGetLastError();
 }
+int scan_chunks(int ntiles) { return ntiles > 0 ? (ntiles + SCAN_CHUNK - 1) / SCAN_CHUNK : 1; }
 hipError_t launch_scan(const Workspace& ws, hipStream_t s) {
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, ws);
+    int chunks = scan_chunks(ws.ntiles);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(chunks), dim3(SCAN_THREADS), 0, s, ws);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_scan_apply, dim3(chunks), dim3(SCAN_THREADS), 0, s, ws);
+    return hipGetLastError();
+}
+hipError_t launch_bin(const PassDesc* d_passes, const Workspace& ws, hipStream_t s) {
+    // grid-stride: the triangle count is only known on the device
+    hipLaunchKernelGGL(k_bin, dim3(4096), dim3(BLOCK), 0, s, d_passes, ws);
     return hipGetLastError();
 }
 hipError_t launch_raster(const PassDesc* d_passes, int npasses, const Workspace& ws, int blocks, hipStream_t s) {
     if (blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_raster, dim3(blocks), dim3(BLOCK), 0, s, d_passes, npasses, ws);
+    hipLaunchKernelGGL(k_raster, dim3(blocks), dim3(RBLOCK), 0, s, d_passes, npasses, ws);
     return hipGetLastError();
 }
 hipError_t launch_present(const uint32_t* src, uint32_t* dst, int w, int h, hipStream_t s) {

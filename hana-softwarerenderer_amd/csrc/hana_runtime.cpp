@@ -133,7 +133,6 @@ int build_plan(hana_ctx* c, std::vector<PassSpec> specs, Plan& pl) {
     pl.fblocks = fb;
     pl.total_v = voff;
     pl.total_vs = vsoff;
-    pl.total_vis = voff / 64;
     pl.total_f = fo;
     pl.total_tiles = tile0;
     pl.ngroups = 0;
@@ -152,19 +151,21 @@ int build_plan(hana_ctx* c, std::vector<PassSpec> specs, Plan& pl) {
 }
 
 // Allocates the arena for a plan and resolves the slice pointers.
-int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_polys, bool with_stats) {
+int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_polys, uint32_t min_tris,
+              bool with_stats) {
     int n = (int)pl.descs.size();
     int r;
     if ((r = ensure(c, A.descs, sizeof(PassDesc) * std::max(n, 1)))) return r;
     if ((r = ensure(c, A.unis, sizeof(DevUniforms) * std::max(n, 1)))) return r;
     if ((r = ensure(c, A.vr, sizeof(float4) * std::max<size_t>(pl.total_v, 1)))) return r;
     if ((r = ensure(c, A.vs, sizeof(float) * std::max<size_t>(pl.total_vs, 1)))) return r;
-    if ((r = ensure(c, A.vis, sizeof(uint64_t) * std::max<size_t>(pl.total_vis, 1)))) return r;
+    if ((r = ensure(c, A.code, std::max<size_t>(pl.total_v, 4)))) return r;
     if ((r = ensure(c, A.pof, sizeof(int32_t) * std::max<size_t>(pl.total_f, 1)))) return r;
     size_t nt = (size_t)pl.total_tiles + 1;
     if ((r = ensure(c, A.tile_count, sizeof(uint32_t) * nt))) return r;
     if ((r = ensure(c, A.tile_offset, sizeof(uint32_t) * nt))) return r;
     if ((r = ensure(c, A.tile_cursor, sizeof(uint32_t) * nt))) return r;
+    if ((r = ensure(c, A.scan_part, sizeof(uint32_t) * (size_t)scan_chunks(pl.total_tiles)))) return r;
     // (triangle, tile) entries: most triangles touch one tile and about half
     // are culled; the scan reports the exact need if this is exceeded
     uint64_t want_bins = std::max<uint64_t>({(uint64_t)min_bins, (uint64_t)4 * pl.total_f + 2 * nt, 1u << 22});
@@ -183,6 +184,14 @@ int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_p
         if ((r = ensure(c, A.pool_n, sizeof(int32_t) * want_polys))) return r;
         A.poly_cap = (uint32_t)want_polys;
     }
+    // listed triangles: at most one per trivially accepted face and eight per
+    // clipped polygon (fan of at most 10 vertices)
+    uint64_t want_tris = std::max<uint64_t>({(uint64_t)min_tris, pl.total_f + 7ull * A.poly_cap, 1024});
+    want_tris = std::min<uint64_t>(want_tris, 0xFFFFFFF0ull);
+    if (want_tris > A.tris_cap) {
+        if ((r = ensure(c, A.tris, sizeof(uint4) * want_tris))) return r;
+        A.tris_cap = (uint32_t)want_tris;
+    }
     // every face may fail the trivial accept (camera inside the mesh)
     uint64_t want_q = std::max<uint64_t>(pl.total_f, 1024);
     want_q = std::min<uint64_t>(want_q, 0xFFFFFFF0ull);
@@ -197,7 +206,7 @@ int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_p
         size_t voff = pl.v_off[i], vsoff = pl.vs_off[i], fo = pl.f_off[i];
         d.vr = A.vr.as<float4>() + voff;
         d.vs = A.vs.as<float>() + vsoff;
-        d.vis = A.vis.as<uint64_t>() + voff / 64;
+        d.code = A.code.as<uint8_t>() + voff;
         d.poly_of_face = A.pof.as<int32_t>() + fo;
         d.u = A.unis.as<DevUniforms>() + i;
         d.stats = with_stats ? A.stats.as<uint64_t>() + (size_t)ST_WORDS * i : nullptr;
@@ -213,6 +222,9 @@ int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_p
     ws.pool_pass = A.pool_pass.as<int32_t>();
     ws.pool_n = A.pool_n.as<int32_t>();
     ws.clip_queue = A.clip_queue.as<uint64_t>();
+    ws.tris = A.tris.as<uint4>();
+    ws.tris_cap = A.tris_cap;
+    ws.scan_part = A.scan_part.as<uint32_t>();
     ws.clipq_cap = A.clipq_cap;
     ws.status = A.status.as<uint32_t>();
     ws.ntiles = pl.total_tiles;
@@ -276,12 +288,12 @@ int enqueue_sequence(hana_ctx* c, const Plan& pl, const Arena& A, hipStream_t s,
     }
     {
         StageTimer t(c, 1, timed, s);
-        HIP_TRY(c, launch_setup(d, n, pl.ws, pl.fblocks, 0, s));
+        HIP_TRY(c, launch_setup(d, n, pl.ws, pl.fblocks, s));
         t.done(s);
     }
     {
         StageTimer t(c, 2, timed, s);
-        HIP_TRY(c, launch_clip(d, pl.ws, 0, s));
+        HIP_TRY(c, launch_clip(d, pl.ws, s));
         t.done(s);
     }
     {
@@ -291,12 +303,7 @@ int enqueue_sequence(hana_ctx* c, const Plan& pl, const Arena& A, hipStream_t s,
     }
     {
         StageTimer t(c, 4, timed, s);
-        HIP_TRY(c, launch_setup(d, n, pl.ws, pl.fblocks, 1, s));
-        t.done(s);
-    }
-    {
-        StageTimer t(c, 5, timed, s);
-        HIP_TRY(c, launch_clip(d, pl.ws, 1, s));
+        HIP_TRY(c, launch_bin(d, pl.ws, s));
         t.done(s);
     }
     for (int g = 0; g < 2; g++)
@@ -351,6 +358,13 @@ int check_status(hana_ctx* c) {
             if (r) return r;
             c->arena.bin_cap = (uint32_t)std::min<uint64_t>(want, 0xFFFFFFF0ull);
         }
+        if (flags & FLAG_TRIS_OVERFLOW) {
+            uint32_t need = c->status_host[STATUS_TRIS];
+            uint64_t want = std::min<uint64_t>((uint64_t)need + need / 4 + 1024, 0xFFFFFFF0ull);
+            int r = ensure(c, c->arena.tris, sizeof(uint4) * want);
+            if (r) return r;
+            c->arena.tris_cap = (uint32_t)want;
+        }
         if (flags & FLAG_POLY_OVERFLOW) {
             uint64_t want = (uint64_t)need_polys + need_polys / 4 + 1024;
             int r;
@@ -379,7 +393,7 @@ int run_passes(hana_ctx* c, std::vector<PassSpec> specs) {
     }
     Plan pl;
     if ((r = build_plan(c, std::move(specs), pl))) return r;
-    if ((r = bind_plan(c, pl, c->arena, 0, 0, c->stats_enabled))) return r;
+    if ((r = bind_plan(c, pl, c->arena, 0, 0, 0, c->stats_enabled))) return r;
     // a target sampled as a shadow map by a pass of the same plan must be drawn
     // by an earlier group; clears fuse into the first pass writing the target
     take_clears(pl);
@@ -445,7 +459,7 @@ int hana_gpu_create(int device, hana_ctx** out) {
 }
 
 static void free_arena(Arena& a) {
-    DevBuf* bufs[] = {&a.descs, &a.unis, &a.vr, &a.vs, &a.vis, &a.pof, &a.tile_count, &a.tile_offset,
+    DevBuf* bufs[] = {&a.descs, &a.unis, &a.vr, &a.vs, &a.code, &a.tris, &a.scan_part, &a.pof, &a.tile_count, &a.tile_offset,
                       &a.tile_cursor, &a.bins, &a.pool_vr, &a.pool_vs, &a.pool_face, &a.pool_pass, &a.pool_n,
                       &a.clip_queue, &a.status, &a.stats};
     for (DevBuf* b : bufs)
@@ -784,9 +798,9 @@ int hana_batch_finalize(hana_ctx* c, hana_batch* b, int use_graph) {
     int r;
     if ((r = check_status(c))) return r;
     if ((r = build_plan(c, b->specs, b->plan))) return r;
-    uint32_t min_bins = 0, min_polys = 0;
+    uint32_t min_bins = 0, min_polys = 0, min_tris = 0;
     for (int attempt = 0; attempt < 3; attempt++) {
-        if ((r = bind_plan(c, b->plan, b->arena, min_bins, min_polys, false))) return r;
+        if ((r = bind_plan(c, b->plan, b->arena, min_bins, min_polys, min_tris, false))) return r;
         batch_clear_flags(b);
         HIP_TRY(c, hipMemcpy(b->arena.descs.p, b->plan.descs.data(), sizeof(PassDesc) * b->plan.descs.size(),
                              hipMemcpyHostToDevice));
@@ -801,6 +815,7 @@ int hana_batch_finalize(hana_ctx* c, hana_batch* b, int use_graph) {
         if (!st[STATUS_FLAGS]) break;
         min_bins = st[STATUS_BINS_NEEDED] + st[STATUS_BINS_NEEDED] / 8 + 1024;
         min_polys = st[STATUS_POLYS] + st[STATUS_POLYS] / 8 + 1024;
+        min_tris = st[STATUS_TRIS] + st[STATUS_TRIS] / 8 + 1024;
         if (attempt == 2) return fail(c, HANA_E_OVERFLOW, "hana_batch_finalize: capacity not reached");
     }
     b->use_graph = use_graph != 0;

@@ -87,14 +87,15 @@ struct hana_server {
 
 namespace {
 
-// Device footprint charged to a request: its targets and an estimate of the
-// per-pass workspace (post-transform vertices, clip bookkeeping, bins).
+// Device footprint charged to a request: its targets and the per-pass
+// workspace (per vertex: position, varyings, outcode; per face: bins, clip
+// queue and pool, triangle list).
 int64_t request_bytes(const hana_request& r, int sw, int sh) {
     int64_t V = r.mesh->V, F = r.mesh->F;
     int passes = r.enable_shadow ? 2 : 1;
     int64_t b = px_bucket((int64_t)r.width * r.height) * 8;
     if (r.enable_shadow) b += px_bucket((int64_t)sw * sh) * 8;
-    b += passes * (V * 56 + F * 32);
+    b += passes * (V * 56 + F * 84);
     return b;
 }
 

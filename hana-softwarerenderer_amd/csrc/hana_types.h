@@ -19,6 +19,15 @@ constexpr int TILE = 16;                   // screen tile edge (pixels)
 constexpr int TILE_SHIFT = 4;
 constexpr int TILE_PIX = TILE * TILE;
 constexpr int BLOCK = 256;                 // threads per workgroup (4 wave64)
+#ifndef HANA_RASTER_BLOCK
+#define HANA_RASTER_BLOCK 256
+#endif
+#ifndef HANA_SMALL_PIX
+#define HANA_SMALL_PIX 16
+#endif
+constexpr int RBLOCK = HANA_RASTER_BLOCK;  // threads per raster workgroup (one tile)
+static_assert(RBLOCK % 64 == 0 && RBLOCK <= TILE_PIX, "raster workgroup: whole waves, at most one lane per pixel");
+static_assert(HANA_SMALL_PIX >= 1 && HANA_SMALL_PIX <= 32, "small-triangle candidates fit a 32-bit mask");
 constexpr int VPT = 4;                     // vertices per thread in k_vertex
 constexpr int FPT = 4;                     // faces per thread in k_setup
 constexpr int POLY_MAX = 10;               // vertices of a clipped polygon (3 + 7 planes)
@@ -71,7 +80,7 @@ struct PassDesc {
     // workspace slices owned by this pass
     float4* vr;                  // per vertex: ndc.x, ndc.y, ndc.z, 1/w
     float* vs;                   // per vertex: nvary varyings
-    uint64_t* vis;               // per 64 vertices: trivially-accept bits
+    uint8_t* code;               // per vertex: bit 7 is_vertex_visible, bit p outside clip plane p
     int32_t* poly_of_face;       // clipped polygon of a face (-1: none)
     uint64_t* stats;             // 5 counters (hana_pass_stats order), or null
     // first block of this pass in each kernel's grid / first global tile
@@ -90,16 +99,31 @@ struct Workspace {
     int32_t* pool_pass;          //   pass of each polygon
     int32_t* pool_n;             //   vertex count of each polygon
     uint64_t* clip_queue;        // faces failing the trivial accept: pass << 32 | face
+    uint4* tris;                 // every triangle with tiles: bin entry, pass, first tile, last tile
+    uint32_t* scan_part;         // per scan chunk: sum of its tile counts
     uint32_t* status;            // see STATUS_*
     int32_t ntiles;
     uint32_t bin_cap;
     uint32_t poly_cap;
     uint32_t clipq_cap;
+    uint32_t tris_cap;
+    uint32_t pad_;
 };
 
-// status words: polygons allocated, overflow flags, bin entries needed, faces queued for clipping
-enum { STATUS_POLYS = 0, STATUS_FLAGS = 1, STATUS_BINS_NEEDED = 2, STATUS_CLIPQ = 3, STATUS_WORDS = 4 };
-enum { FLAG_BIN_OVERFLOW = 1, FLAG_POLY_OVERFLOW = 2, FLAG_POLY_TOO_LARGE = 4, FLAG_CLIPQ_OVERFLOW = 8 };
+// the tile scan: SCAN_CHUNK tile counters per workgroup of SCAN_THREADS
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_PER_THREAD = 16;
+constexpr int SCAN_CHUNK = SCAN_THREADS * SCAN_PER_THREAD;
+
+// status words: polygons allocated, overflow flags, bin entries needed, faces
+// queued for clipping, triangles listed for binning
+enum { STATUS_POLYS = 0, STATUS_FLAGS = 1, STATUS_BINS_NEEDED = 2, STATUS_CLIPQ = 3, STATUS_TRIS = 4,
+       STATUS_WORDS = 5 };
+enum { FLAG_BIN_OVERFLOW = 1, FLAG_POLY_OVERFLOW = 2, FLAG_POLY_TOO_LARGE = 4, FLAG_CLIPQ_OVERFLOW = 8,
+       FLAG_TRIS_OVERFLOW = 16 };
+// vertex outcode bits
+constexpr uint32_t CODE_VISIBLE = 0x80u;
+constexpr uint32_t CODE_PLANES = 0x7Fu;
 enum { ST_FACES = 0, ST_CLIPPED = 1, ST_POLYS = 2, ST_ENTRIES = 3, ST_PIXELS = 4, ST_WORDS = 5 };
 
 }  // namespace hana

@@ -275,8 +275,7 @@ class Context:
         self.check(self.lib.hana_gpu_last_stats(self.h, arr, max_passes, C.byref(n)))
         return [arr[i].as_dict() for i in range(n.value)]
 
-    STAGES = ("vertex", "setup_count", "clip_count", "scan", "setup_scatter", "clip_scatter", "raster_g0",
-              "raster_g1")
+    STAGES = ("vertex", "setup", "clip", "scan", "bin", "unused", "raster_g0", "raster_g1")
 
     def set_timing(self, on: bool):
         self.check(self.lib.hana_gpu_set_timing(self.h, int(on)))

@@ -121,11 +121,11 @@ int hana_gpu_set_stats(hana_ctx* ctx, int enable);
 int hana_gpu_last_stats(hana_ctx* ctx, hana_pass_stats* out, int max_passes, int* n_passes);
 
 /* Per-stage kernel timing with HIP events recorded on the context stream
- * around each launch (benchmarking).  Stages: 0 vertex, 1 set-up/count,
- * 2 clip/count, 3 scan, 4 set-up/scatter, 5 clip/scatter, 6 raster of the
- * first dependency group (shadow passes), 7 raster of the second group (main
- * passes).  Reading returns the milliseconds and launch counts accumulated
- * since the last read. */
+ * around each launch (benchmarking).  Stages: 0 vertex, 1 set-up (cull,
+ * bounds, tile counts), 2 clipping, 3 tile scan, 4 binning, 5 (unused, no
+ * launches), 6 raster of the first dependency group (shadow passes), 7 raster
+ * of the second group (main passes).  Reading returns the milliseconds and
+ * launch counts accumulated since the last read. */
 #define HANA_TIMING_STAGES 8
 int hana_gpu_set_timing(hana_ctx* ctx, int enable);
 int hana_gpu_read_timing(hana_ctx* ctx, double* ms, int64_t* launches);
