@@ -263,6 +263,23 @@ def bench_micro(ctx, args):
     return out
 
 
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic_large_frame.json")
+TRAFFIC_SOURCE = ("profiles/counters_large_frame.md: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch, "
+                  "scaled by a plain copy's readings")
+
+
+def measured_traffic(stage):
+    """HBM bytes per launch of a stage from the committed counter summary of
+    this workload (None if absent)."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    per = json.load(open(TRAFFIC_FILE)).get("per_launch", {})
+    for k, v in per.items():
+        if k.split(" ")[0] == stage:
+            return int(v["read_bytes"] + v["write_bytes"])
+    return None
+
+
 def alg_bytes(scene):
     """Algorithmic HBM bytes of one frame, split by stage:
     a2v 32 B and indices 12 B per vertex/triangle per pass, colour+depth 8 B
@@ -366,6 +383,7 @@ def main():
 
     per_stage_bytes, frame_bytes = alg_bytes(scene)
     dom = max(stage_ms, key=lambda k: stage_ms[k])
+    traffic = measured_traffic(dom)
     dom_gbps = per_stage_bytes.get(dom, 0) / (stage_ms[dom] * 1e-3) / 1e9 if stage_ms[dom] > 0 else 0.0
     value = frags * args.steps / elapsed / 1e6 if frags else None
     res = {
@@ -390,7 +408,8 @@ def main():
         "stage_ms": {k: round(v, 5) for k, v in stage_ms.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(dom_gbps, 2), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(dom_gbps / HBM_PEAK_GBPS, 5),
-                     "bytes_per_launch": per_stage_bytes.get(dom, 0), "traffic": None,
+                     "bytes_per_launch": per_stage_bytes.get(dom, 0), "traffic": traffic,
+                     "traffic_source": TRAFFIC_SOURCE if traffic is not None else None,
                      "frame_alg_bytes": frame_bytes,
                      "frame_achieved": round(frame_bytes / (ms_per_step * 1e-3) / 1e9, 2)},
         "parity": parity,

@@ -53,7 +53,7 @@ struct PassSpec {
 
 // Grow-only device workspace for launch sequences.
 struct Arena {
-    DevBuf descs, unis, vr, vs, code, pof, tile_count, tile_offset, tile_cursor, bins, tris, scan_part;
+    DevBuf descs, unis, vr, vs, code, pof, tile_count, tile_offset, tile_cursor, bins, tris, seg_count, scan_part;
     DevBuf pool_vr, pool_vs, pool_face, pool_pass, pool_n, clip_queue, status, stats;
     uint32_t bin_cap = 0, poly_cap = 0, clipq_cap = 0, tris_cap = 0;
 };

@@ -304,25 +304,40 @@ __device__ __forceinline__ int clip_stage(int plane, int n, LanePoly<NF> in, Lan
     }
     return m;
 }
-// Returns the vertex count (result in b), 0 when fewer than 3 remain, -1 on
-// a polygon beyond POLY_MAX vertices.
+// True when every vertex of the polygon is inside the plane: clip_stage would
+// then return the same vertices in the same order, so the stage is skipped.
 template <int NF>
-__device__ __forceinline__ int clip_polygon(LanePoly<NF> a, LanePoly<NF> b) {
+__device__ __forceinline__ bool all_inside(int plane, int n, LanePoly<NF> p) {
+    for (int i = 0; i < n; i++) {
+        float c[4];
+        for (int k = 0; k < 4; k++) c[k] = p.at(i, k);
+        if (!inside_plane(c, plane)) return false;
+    }
+    return true;
+}
+
+// The 7 planes in order on the triangle in `cur`.  Returns the vertex count
+// with the result in `cur` (the buffers swap per stage), 0 when fewer than 3
+// remain, -1 for a polygon beyond POLY_MAX vertices.
+template <int NF>
+__device__ __forceinline__ int clip_polygon(LanePoly<NF>& cur, LanePoly<NF>& other) {
     int n = 3;
-    n = clip_stage<NF>(PL_W, n, a, b); if (n < 3) return n < 0 ? -1 : 0;
-    n = clip_stage<NF>(PL_PX, n, b, a); if (n < 3) return n < 0 ? -1 : 0;
-    n = clip_stage<NF>(PL_NX, n, a, b); if (n < 3) return n < 0 ? -1 : 0;
-    n = clip_stage<NF>(PL_PY, n, b, a); if (n < 3) return n < 0 ? -1 : 0;
-    n = clip_stage<NF>(PL_NY, n, a, b); if (n < 3) return n < 0 ? -1 : 0;
-    n = clip_stage<NF>(PL_PZ, n, b, a); if (n < 3) return n < 0 ? -1 : 0;
-    n = clip_stage<NF>(PL_NZ, n, a, b); if (n < 3) return n < 0 ? -1 : 0;
+    for (int plane = PL_W; plane <= PL_NZ; plane++) {
+        if (all_inside<NF>(plane, n, cur)) continue;
+        n = clip_stage<NF>(plane, n, cur, other);
+        if (n < 3) return n < 0 ? -1 : 0;
+        LanePoly<NF> t = cur;
+        cur = other;
+        other = t;
+    }
     return n;
 }
 
-// Re-runs the vertex program on the face's corners, clips, and stores the
-// polygon (perspective-divided) in the pool.  Returns the polygon id or -1.
+// Re-runs the vertex program on the face's corners and clips the triangle in
+// the lane's LDS buffers.  Returns the vertex count (>= 3) with the polygon in
+// buffer b when *in_b, else in a; 0 when nothing remains.
 // lds: 2 * POLY_MAX * CLIP_NF * CLIP_LANES floats shared by the wave.
-__device__ __noinline__ int clip_face(const PassDesc& P, const Workspace& ws, int pass, int f, float* lds, int lane) {
+__device__ __noinline__ int clip_face(const PassDesc& P, const Workspace& ws, int f, float* lds, int lane, int* in_b) {
     const DevUniforms& u = *P.u;
     LanePoly<CLIP_NF> a{lds, lane};
     LanePoly<CLIP_NF> b{lds + POLY_MAX * CLIP_NF * CLIP_LANES, lane};
@@ -337,8 +352,8 @@ __device__ __noinline__ int clip_face(const PassDesc& P, const Workspace& ws, in
         for (int c = 0; c < 4; c++) a.at(k, c) = clip[c];
         for (int c = 0; c < POLY_VARY; c++) a.at(k, 4 + c) = vary[c];
     }
-    // positions only first (in b's space): many such faces lie wholly
-    // outside and are dropped without interpolating varyings
+    // positions only first (in b's space): faces that vanish are dropped
+    // without interpolating varyings
     {
         LanePoly<4> pa{b.base, lane};
         LanePoly<4> pb{b.base + POLY_MAX * 4 * CLIP_LANES, lane};
@@ -346,29 +361,30 @@ __device__ __noinline__ int clip_face(const PassDesc& P, const Workspace& ws, in
             for (int c = 0; c < 4; c++) pa.at(k, c) = a.at(k, c);
         int np = clip_polygon<4>(pa, pb);
         if (np < 0) atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_POLY_TOO_LARGE);
-        if (np < 3) return -1;
+        if (np < 3) return 0;
     }
-    int n = clip_polygon<CLIP_NF>(a, b);
-    if (n < 3) return -1;
-    int pid = (int)atomicAdd(&ws.status[STATUS_POLYS], 1u);
-    if ((uint32_t)pid >= ws.poly_cap) {
-        atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_POLY_OVERFLOW);
-        return -1;
-    }
+    LanePoly<CLIP_NF> cur = a, oth = b;
+    int n = clip_polygon<CLIP_NF>(cur, oth);
+    if (n < 3) return 0;
+    *in_b = cur.base == b.base;
+    return n;
+}
+
+// Stores a clipped polygon (perspective-divided) in the pool slot pid.
+__device__ void store_polygon(const Workspace& ws, LanePoly<CLIP_NF> cur, int n, int pid, int pass, int f) {
     for (int k = 0; k < n; k++) {
-        float w = b.at(k, 3);
+        float w = cur.at(k, 3);
         float4 r;
         r.w = 1 / w;
-        r.z = b.at(k, 2) / w;
-        r.y = b.at(k, 1) / w;
-        r.x = b.at(k, 0) / w;
+        r.z = cur.at(k, 2) / w;
+        r.y = cur.at(k, 1) / w;
+        r.x = cur.at(k, 0) / w;
         ws.pool_vr[(size_t)pid * POLY_MAX + k] = r;
-        for (int c = 0; c < POLY_VARY; c++) ws.pool_vs[((size_t)pid * POLY_MAX + k) * POLY_VARY + c] = b.at(k, 4 + c);
+        for (int c = 0; c < POLY_VARY; c++) ws.pool_vs[((size_t)pid * POLY_MAX + k) * POLY_VARY + c] = cur.at(k, 4 + c);
     }
     ws.pool_face[pid] = f;
     ws.pool_pass[pid] = pass;
     ws.pool_n[pid] = n;
-    return pid;
 }
 
 // ---------------------------------------------------------------------------
@@ -401,10 +417,28 @@ __device__ __forceinline__ uint32_t wave_reserve(uint32_t* counters, int t, bool
     return mine;
 }
 
-// Counts every (triangle, tile) pair of one lane's triangle and lists the
-// triangle (bin entry, pass, first and last tile) for k_bin; wave-uniform.
+// Adds one to counters[t] for every lane with `want`: one atomic per distinct
+// t in the wave, none of them waited for.  Wave-uniform like wave_reserve.
+__device__ __forceinline__ void wave_count(uint32_t* counters, int t, bool want) {
+    bool pending = want;
+    uint64_t mask = __ballot(pending);
+    int lane = __lane_id();
+    while (mask) {
+        int leader = __ffsll((unsigned long long)mask) - 1;
+        int lt = __shfl(t, leader);
+        bool match = pending && t == lt;
+        uint64_t same = __ballot(match);
+        if (lane == leader) atomicAdd(&counters[lt], (uint32_t)__popcll(same));
+        if (match) pending = false;
+        mask = __ballot(pending);
+    }
+}
+
+// Counts every (triangle, tile) pair of one lane's triangle and writes its
+// record (bin entry, pass, first and last tile) for k_bin to ws.tris[slot];
+// wave-uniform.
 __device__ __forceinline__ void count_tiles(const PassDesc& P, const Workspace& ws, int pass, bool ok, const TriS& t,
-                                            uint32_t entry) {
+                                            uint32_t entry, uint32_t slot) {
     int tx0 = 0, ty0 = 0, tx1 = 0, ty1 = 0, nx = 0, n = 0;
     if (ok) {
         tx0 = t.x0 >> TILE_SHIFT;
@@ -413,24 +447,30 @@ __device__ __forceinline__ void count_tiles(const PassDesc& P, const Workspace& 
         ty1 = t.y1 >> TILE_SHIFT;
         nx = tx1 - tx0 + 1;
         n = nx * (ty1 - ty0 + 1);
-    }
-    uint32_t slot = wave_reserve(&ws.status[STATUS_TRIS], 0, ok);
-    if (ok) {
-        if (slot < ws.tris_cap)
-            ws.tris[slot] = make_uint4(entry, (uint32_t)pass, (uint32_t)tx0 | ((uint32_t)ty0 << 16),
-                                       (uint32_t)tx1 | ((uint32_t)ty1 << 16));
-        else
-            atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_TRIS_OVERFLOW);
+        ws.tris[slot] = make_uint4(entry, (uint32_t)pass, (uint32_t)tx0 | ((uint32_t)ty0 << 16),
+                                   (uint32_t)tx1 | ((uint32_t)ty1 << 16));
     }
     for (int k = 0; __ballot(k < n); k++) {
         bool want = k < n;
         int gt = want ? P.tile0 + (ty0 + k / nx) * P.tiles_x + tx0 + k % nx : 0;
-        wave_reserve(ws.tile_count, gt, want);
+        wave_count(ws.tile_count, gt, want);
     }
     if (P.stats) {
         for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off);
         if (__lane_id() == 0) stat_add(P, ST_ENTRIES, (uint64_t)n);
     }
+}
+
+// A slot in an LDS counter for every lane with `want`, one LDS atomic per
+// wave; wave-uniform.
+__device__ __forceinline__ uint32_t wave_reserve_lds(uint32_t* counter, bool want) {
+    uint64_t m = __ballot(want);
+    int lane = __lane_id();
+    uint32_t base = 0;
+    int leader = m ? __ffsll((unsigned long long)m) - 1 : 0;
+    if (m && lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
 }
 
 // Faces that fail the trivial accept, from the outcodes of their vertices:
@@ -446,12 +486,19 @@ __device__ __forceinline__ bool dies_at_first_crossed_plane(uint32_t c0, uint32_
 }
 
 // FPT faces per lane.  Trivially accepted faces are set up from the shared
-// post-transform vertices; faces that clipping would remove are dropped; the
-// others are queued for k_clip.
+// post-transform vertices and listed in this workgroup's segment of ws.tris
+// (BLOCK * FPT records from blockIdx.x * BLOCK * FPT; the count goes to
+// ws.seg_count); faces that clipping would remove are dropped; the others are
+// queued for k_clip with one global atomic per workgroup.
 __global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* passes, int npasses, Workspace ws) {
+    __shared__ uint64_t s_queue[BLOCK * FPT];
+    __shared__ uint32_t s_ntri, s_nq, s_qbase;
+    if (threadIdx.x == 0) { s_ntri = 0; s_nq = 0; }
+    __syncthreads();
     int pass = find_pass<1>(passes, npasses, blockIdx.x);
     const PassDesc& P = passes[pass];
     int base = (blockIdx.x - P.fblock0) * BLOCK * FPT;
+    uint32_t seg = (uint32_t)blockIdx.x * (BLOCK * FPT);
     int ids[FPT][3];
     bool valid[FPT];
 #pragma unroll
@@ -486,11 +533,8 @@ __global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* passes, int npa
         bool clipped = valid[k] && !trivial[k];
         bool clipq = clipped && !dies_at_first_crossed_plane(codes[k][0], codes[k][1], codes[k][2]);
         if (clipped && !clipq) P.poly_of_face[f] = -1;
-        uint32_t q = wave_reserve(&ws.status[STATUS_CLIPQ], 0, clipq);
-        if (clipq) {
-            if (q < ws.clipq_cap) ws.clip_queue[q] = ((uint64_t)(uint32_t)pass << 32) | (uint32_t)f;
-            else atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_CLIPQ_OVERFLOW);
-        }
+        uint32_t q = wave_reserve_lds(&s_nq, clipq);
+        if (clipq) s_queue[q] = ((uint64_t)(uint32_t)pass << 32) | (uint32_t)f;
         if (P.stats) {
             uint64_t nv = __popcll(__ballot(valid[k])), nc = __popcll(__ballot(clipped));
             if (__lane_id() == 0) {
@@ -500,64 +544,108 @@ __global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* passes, int npa
         }
         TriS t;
         bool ok = trivial[k] && tri_setup(va[k], vb[k], vc[k], P.width, P.height, t);
-        count_tiles(P, ws, pass, ok, t, (uint32_t)f);
+        uint32_t slot = wave_reserve_lds(&s_ntri, ok);
+        count_tiles(P, ws, pass, ok, t, (uint32_t)f, seg + slot);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ws.seg_count[blockIdx.x] = s_ntri;
+        s_qbase = s_nq ? atomicAdd(&ws.status[STATUS_CLIPQ], s_nq) : 0u;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < s_nq; i += BLOCK) {
+        uint32_t q = s_qbase + i;
+        if (q < ws.clipq_cap) ws.clip_queue[q] = s_queue[i];
+        else atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_CLIPQ_OVERFLOW);
     }
 }
 
 // Clips the queued faces: one wave per workgroup, grid-stride over the queue,
-// polygon buffers in LDS; counts and lists the fan triangles (0, j+1, j+2).
+// polygon buffers in LDS; pool slots and the fan triangles' records (0, j+1,
+// j+2; after the setup segments of ws.tris) are reserved once per wave.
 __global__ __launch_bounds__(CLIP_LANES) void k_clip(const PassDesc* passes, Workspace ws) {
     __shared__ float lds[2 * POLY_MAX * CLIP_NF * CLIP_LANES];
     uint32_t count = min(ws.status[STATUS_CLIPQ], ws.clipq_cap);
+    uint32_t clip_cap = ws.tris_cap - ws.seg_region;
     for (uint32_t base = blockIdx.x * CLIP_LANES; base < count; base += gridDim.x * CLIP_LANES) {
         uint32_t q = base + threadIdx.x;
         bool have = q < count;
-        int pass = 0, f = 0, pid = -1;
+        int pass = 0, f = 0, n = 0, in_b = 0;
         if (have) {
             uint64_t e = ws.clip_queue[q];
             pass = (int)(e >> 32);
             f = (int)(uint32_t)e;
-            pid = clip_face(passes[pass], ws, pass, f, lds, threadIdx.x);
-            passes[pass].poly_of_face[f] = pid;
-            if (pid >= 0) stat_add(passes[pass], ST_POLYS, 1);
+            n = clip_face(passes[pass], ws, f, lds, threadIdx.x, &in_b);
         }
-        int n = pid >= 0 ? ws.pool_n[pid] : 0;
-        for (int j = 0; __ballot(j < n - 2); j++) {
+        bool keep = n >= 3;
+        uint32_t pid = wave_reserve(&ws.status[STATUS_POLYS], 0, keep);
+        if (keep && pid >= ws.poly_cap) {
+            atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_POLY_OVERFLOW);
+            keep = false;
+        }
+        if (keep) {
+            LanePoly<CLIP_NF> cur{lds + (in_b ? POLY_MAX * CLIP_NF * CLIP_LANES : 0), (int)threadIdx.x};
+            store_polygon(ws, cur, n, (int)pid, pass, f);
+            stat_add(passes[pass], ST_POLYS, 1);
+        }
+        if (have) passes[pass].poly_of_face[f] = keep ? (int)pid : -1;
+        int nt = keep ? n - 2 : 0;
+        for (int j = 0; __ballot(j < nt); j++) {
             TriS t;
             bool ok = false;
-            if (j < n - 2) {
+            if (j < nt) {
                 const float4* pv = ws.pool_vr + (size_t)pid * POLY_MAX;
                 ok = tri_setup(pv[0], pv[j + 1], pv[j + 2], passes[pass].width, passes[pass].height, t);
             }
-            count_tiles(passes[pass], ws, pass, ok, t, CLIPPED_BIT | ((uint32_t)max(pid, 0) << 3) | (uint32_t)j);
+            uint32_t slot = wave_reserve(&ws.status[STATUS_TRIS], 0, ok);
+            if (ok && slot >= clip_cap) {
+                atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_TRIS_OVERFLOW);
+                ok = false;
+            }
+            count_tiles(passes[pass], ws, pass, ok, t, CLIPPED_BIT | (pid << 3) | (uint32_t)j, ws.seg_region + slot);
         }
     }
 }
 
-// Scatters every listed triangle into the bins of its tiles; grid-stride.
-__global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* passes, Workspace ws) {
-    uint32_t count = min(ws.status[STATUS_TRIS], ws.tris_cap);
-    for (uint32_t base = blockIdx.x * BLOCK; base < count; base += gridDim.x * BLOCK) {
+// Scatters one listed triangle into the bins of its tiles; wave-uniform.
+__device__ __forceinline__ void bin_record(const PassDesc* passes, const Workspace& ws, bool have, uint4 r) {
+    int tx0 = 0, ty0 = 0, nx = 0, n = 0, tile0 = 0, tiles_x = 0;
+    if (have) {
+        const PassDesc& P = passes[r.y];
+        tx0 = (int)(r.z & 0xFFFFu);
+        ty0 = (int)(r.z >> 16);
+        nx = (int)(r.w & 0xFFFFu) - tx0 + 1;
+        n = nx * ((int)(r.w >> 16) - ty0 + 1);
+        tile0 = P.tile0;
+        tiles_x = P.tiles_x;
+    }
+    for (int k = 0; __ballot(k < n); k++) {
+        bool want = k < n;
+        int gt = want ? tile0 + (ty0 + k / nx) * tiles_x + tx0 + k % nx : 0;
+        uint32_t pos = wave_reserve(ws.tile_cursor, gt, want);
+        if (want && pos < ws.bin_cap) ws.bins[pos] = r.x;
+    }
+}
+
+// Workgroups below nseg scatter the setup segment of their index; the others
+// stride over the clipped fan triangles listed after the segments.
+__global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* passes, Workspace ws, int nseg) {
+    if ((int)blockIdx.x < nseg) {
+        uint32_t n = ws.seg_count[blockIdx.x];
+        const uint4* rec = ws.tris + (size_t)blockIdx.x * (BLOCK * FPT);
+        for (uint32_t base = 0; base < n; base += BLOCK) {
+            uint32_t i = base + threadIdx.x;
+            bool have = i < n;
+            bin_record(passes, ws, have, have ? rec[i] : make_uint4(0, 0, 0, 0));
+        }
+        return;
+    }
+    uint32_t count = min(ws.status[STATUS_TRIS], ws.tris_cap - ws.seg_region);
+    uint32_t stride = (gridDim.x - (uint32_t)nseg) * BLOCK;
+    for (uint32_t base = (blockIdx.x - (uint32_t)nseg) * BLOCK; base < count; base += stride) {
         uint32_t i = base + threadIdx.x;
-        int tx0 = 0, ty0 = 0, nx = 0, n = 0, tile0 = 0, tiles_x = 0;
-        uint32_t entry = 0;
-        if (i < count) {
-            uint4 r = ws.tris[i];
-            const PassDesc& P = passes[r.y];
-            entry = r.x;
-            tx0 = (int)(r.z & 0xFFFFu);
-            ty0 = (int)(r.z >> 16);
-            nx = (int)(r.w & 0xFFFFu) - tx0 + 1;
-            n = nx * ((int)(r.w >> 16) - ty0 + 1);
-            tile0 = P.tile0;
-            tiles_x = P.tiles_x;
-        }
-        for (int k = 0; __ballot(k < n); k++) {
-            bool want = k < n;
-            int gt = want ? tile0 + (ty0 + k / nx) * tiles_x + tx0 + k % nx : 0;
-            uint32_t pos = wave_reserve(ws.tile_cursor, gt, want);
-            if (want && pos < ws.bin_cap) ws.bins[pos] = entry;
-        }
+        bool have = i < count;
+        bin_record(passes, ws, have, have ? ws.tris[ws.seg_region + i] : make_uint4(0, 0, 0, 0));
     }
 }
 
@@ -636,6 +724,12 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(Workspace ws) {
 
 constexpr int SMALL_PIX = HANA_SMALL_PIX;   // triangles with at most this many candidate pixels: one lane each
 
+// k / w for 0 <= k < TILE_PIX and 1 <= w <= TILE as (k * magic) >> 16 with
+// magic = ceil(2^16 / w): the excess k * (magic - 2^16/w) / 2^16 stays below
+// 1/256, less than the gap 1/w between k/w and the next integer.
+__device__ __forceinline__ uint32_t tile_div_magic(int w) { return (65536u + (uint32_t)w - 1u) / (uint32_t)w; }
+__device__ __forceinline__ int tile_div(int k, uint32_t magic) { return (int)(((uint32_t)k * magic) >> 16); }
+
 struct TileLds {
     uint64_t key[TILE_PIX];
     uint32_t nan_order[TILE_PIX];   // 1 + order of the last NaN-depth fragment
@@ -705,15 +799,24 @@ __device__ void sweep(TileLds& L, const PassDesc& P, const Workspace& ws, uint32
             int wdt = bx1 - bx0 + 1;
             int npix = wdt * (by1 - by0 + 1);
             if (npix <= SMALL_PIX) {
+#if HANA_EARLY_REJECT
                 // cheap pass over the candidates, then the exact test on the survivors
                 uint32_t mask = 0;
-                for (int k = 0; k < npix; k++)
-                    if (maybe_covered(t, xb + bx0 + k % wdt, yb + by0 + k / wdt)) mask |= 1u << k;
+                int k = 0;
+                for (int ly = by0; ly <= by1; ly++)
+                    for (int lx = bx0; lx <= bx1; lx++, k++)
+                        if (maybe_covered(t, xb + lx, yb + ly)) mask |= 1u << k;
+                uint32_t magic = tile_div_magic(wdt);
                 while (mask) {
-                    int k = __builtin_ctz(mask);
+                    int b = __builtin_ctz(mask);
                     mask &= mask - 1u;
-                    frag_resolve(L, t, lo, xb, yb, bx0 + k % wdt, by0 + k / wdt, second);
+                    int row = tile_div(b, magic);
+                    frag_resolve(L, t, lo, xb, yb, bx0 + b - row * wdt, by0 + row, second);
                 }
+#else
+                for (int ly = by0; ly <= by1; ly++)
+                    for (int lx = bx0; lx <= bx1; lx++) frag_resolve(L, t, lo, xb, yb, lx, ly, second);
+#endif
             } else {
                 L.ax[tid] = t.ax; L.ay[tid] = t.ay; L.s0x[tid] = t.s0x; L.s0y[tid] = t.s0y;
                 L.s1x[tid] = t.s1x; L.s1y[tid] = t.s1y; L.uz[tid] = t.uz;
@@ -739,9 +842,10 @@ __device__ void sweep(TileLds& L, const PassDesc& P, const Workspace& ws, uint32
             int x0 = box & 0xFF, x1 = (box >> 8) & 0xFF, y0 = (box >> 16) & 0xFF, y1 = box >> 24;
             int wdt = x1 - x0 + 1;
             int npix = wdt * (y1 - y0 + 1);
+            uint32_t magic = tile_div_magic(wdt);
             for (int k = lane; k < npix; k += 64) {
-                int ly = y0 + k / wdt, lx = x0 + k % wdt;
-                if (maybe_covered(bt, xb + lx, yb + ly)) frag_resolve(L, bt, blo, xb, yb, lx, ly, second);
+                int row = tile_div(k, magic);
+                frag_resolve(L, bt, blo, xb, yb, x0 + k - row * wdt, y0 + row, second);
             }
         }
         __syncthreads();
@@ -914,9 +1018,10 @@ hipError_t launch_scan(const Workspace& ws, hipStream_t s) {
     hipLaunchKernelGGL(k_scan_apply, dim3(chunks), dim3(SCAN_THREADS), 0, s, ws);
     return hipGetLastError();
 }
-hipError_t launch_bin(const PassDesc* d_passes, const Workspace& ws, hipStream_t s) {
-    // grid-stride: the triangle count is only known on the device
-    hipLaunchKernelGGL(k_bin, dim3(4096), dim3(BLOCK), 0, s, d_passes, ws);
+hipError_t launch_bin(const PassDesc* d_passes, const Workspace& ws, int nseg, hipStream_t s) {
+    // one workgroup per setup segment, then a grid-stride tail for the
+    // clipped fan triangles (their count is only known on the device)
+    hipLaunchKernelGGL(k_bin, dim3(nseg + 512), dim3(BLOCK), 0, s, d_passes, ws, nseg);
     return hipGetLastError();
 }
 hipError_t launch_raster(const PassDesc* d_passes, int npasses, const Workspace& ws, int blocks, hipStream_t s) {

@@ -184,14 +184,16 @@ int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_p
         if ((r = ensure(c, A.pool_n, sizeof(int32_t) * want_polys))) return r;
         A.poly_cap = (uint32_t)want_polys;
     }
-    // listed triangles: at most one per trivially accepted face and eight per
-    // clipped polygon (fan of at most 10 vertices)
-    uint64_t want_tris = std::max<uint64_t>({(uint64_t)min_tris, pl.total_f + 7ull * A.poly_cap, 1024});
-    want_tris = std::min<uint64_t>(want_tris, 0xFFFFFFF0ull);
+    // listed triangles: a segment of BLOCK*FPT records per k_setup workgroup,
+    // then at most eight fan triangles per clipped polygon
+    uint64_t seg_region = (uint64_t)pl.fblocks * BLOCK * FPT;
+    uint64_t want_tris = seg_region + std::max<uint64_t>({(uint64_t)min_tris, 8ull * A.poly_cap, 1024});
+    if (want_tris > 0xFFFFFFF0ull) return fail(c, HANA_E_NOMEM, "launch sequence too large (triangle list)");
     if (want_tris > A.tris_cap) {
         if ((r = ensure(c, A.tris, sizeof(uint4) * want_tris))) return r;
         A.tris_cap = (uint32_t)want_tris;
     }
+    if ((r = ensure(c, A.seg_count, sizeof(uint32_t) * std::max(pl.fblocks, 1)))) return r;
     // every face may fail the trivial accept (camera inside the mesh)
     uint64_t want_q = std::max<uint64_t>(pl.total_f, 1024);
     want_q = std::min<uint64_t>(want_q, 0xFFFFFFF0ull);
@@ -224,6 +226,8 @@ int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_p
     ws.clip_queue = A.clip_queue.as<uint64_t>();
     ws.tris = A.tris.as<uint4>();
     ws.tris_cap = A.tris_cap;
+    ws.seg_region = (uint32_t)seg_region;
+    ws.seg_count = A.seg_count.as<uint32_t>();
     ws.scan_part = A.scan_part.as<uint32_t>();
     ws.clipq_cap = A.clipq_cap;
     ws.status = A.status.as<uint32_t>();
@@ -303,7 +307,7 @@ int enqueue_sequence(hana_ctx* c, const Plan& pl, const Arena& A, hipStream_t s,
     }
     {
         StageTimer t(c, 4, timed, s);
-        HIP_TRY(c, launch_bin(d, pl.ws, s));
+        HIP_TRY(c, launch_bin(d, pl.ws, pl.fblocks, s));
         t.done(s);
     }
     for (int g = 0; g < 2; g++)
@@ -360,7 +364,7 @@ int check_status(hana_ctx* c) {
         }
         if (flags & FLAG_TRIS_OVERFLOW) {
             uint32_t need = c->status_host[STATUS_TRIS];
-            uint64_t want = std::min<uint64_t>((uint64_t)need + need / 4 + 1024, 0xFFFFFFF0ull);
+            uint64_t want = std::min<uint64_t>((uint64_t)c->arena.tris_cap + need + need / 4 + 1024, 0xFFFFFFF0ull);
             int r = ensure(c, c->arena.tris, sizeof(uint4) * want);
             if (r) return r;
             c->arena.tris_cap = (uint32_t)want;
@@ -459,7 +463,7 @@ int hana_gpu_create(int device, hana_ctx** out) {
 }
 
 static void free_arena(Arena& a) {
-    DevBuf* bufs[] = {&a.descs, &a.unis, &a.vr, &a.vs, &a.code, &a.tris, &a.scan_part, &a.pof, &a.tile_count, &a.tile_offset,
+    DevBuf* bufs[] = {&a.descs, &a.unis, &a.vr, &a.vs, &a.code, &a.tris, &a.seg_count, &a.scan_part, &a.pof, &a.tile_count, &a.tile_offset,
                       &a.tile_cursor, &a.bins, &a.pool_vr, &a.pool_vs, &a.pool_face, &a.pool_pass, &a.pool_n,
                       &a.clip_queue, &a.status, &a.stats};
     for (DevBuf* b : bufs)

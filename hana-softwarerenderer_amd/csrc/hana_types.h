@@ -23,7 +23,10 @@ constexpr int BLOCK = 256;                 // threads per workgroup (4 wave64)
 #define HANA_RASTER_BLOCK 256
 #endif
 #ifndef HANA_SMALL_PIX
-#define HANA_SMALL_PIX 16
+#define HANA_SMALL_PIX 32
+#endif
+#ifndef HANA_EARLY_REJECT
+#define HANA_EARLY_REJECT 1
 #endif
 constexpr int RBLOCK = HANA_RASTER_BLOCK;  // threads per raster workgroup (one tile)
 static_assert(RBLOCK % 64 == 0 && RBLOCK <= TILE_PIX, "raster workgroup: whole waves, at most one lane per pixel");
@@ -99,7 +102,9 @@ struct Workspace {
     int32_t* pool_pass;          //   pass of each polygon
     int32_t* pool_n;             //   vertex count of each polygon
     uint64_t* clip_queue;        // faces failing the trivial accept: pass << 32 | face
-    uint4* tris;                 // every triangle with tiles: bin entry, pass, first tile, last tile
+    uint4* tris;                 // every triangle with tiles: bin entry, pass, first tile, last tile;
+                                 //   BLOCK*FPT per k_setup workgroup, then the clipped fan triangles
+    uint32_t* seg_count;         // records in each k_setup workgroup's segment
     uint32_t* scan_part;         // per scan chunk: sum of its tile counts
     uint32_t* status;            // see STATUS_*
     int32_t ntiles;
@@ -107,7 +112,7 @@ struct Workspace {
     uint32_t poly_cap;
     uint32_t clipq_cap;
     uint32_t tris_cap;
-    uint32_t pad_;
+    uint32_t seg_region;         // records reserved for the setup segments
 };
 
 // the tile scan: SCAN_CHUNK tile counters per workgroup of SCAN_THREADS
@@ -116,7 +121,7 @@ constexpr int SCAN_PER_THREAD = 16;
 constexpr int SCAN_CHUNK = SCAN_THREADS * SCAN_PER_THREAD;
 
 // status words: polygons allocated, overflow flags, bin entries needed, faces
-// queued for clipping, triangles listed for binning
+// queued for clipping, clipped fan triangles listed for binning
 enum { STATUS_POLYS = 0, STATUS_FLAGS = 1, STATUS_BINS_NEEDED = 2, STATUS_CLIPQ = 3, STATUS_TRIS = 4,
        STATUS_WORDS = 5 };
 enum { FLAG_BIN_OVERFLOW = 1, FLAG_POLY_OVERFLOW = 2, FLAG_POLY_TOO_LARGE = 4, FLAG_CLIPQ_OVERFLOW = 8,
