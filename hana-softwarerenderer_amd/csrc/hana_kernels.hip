@@ -627,16 +627,72 @@ __device__ __forceinline__ void bin_record(const PassDesc* passes, const Workspa
     }
 }
 
-// Workgroups below nseg scatter the setup segment of their index; the others
-// stride over the clipped fan triangles listed after the segments.
+// The tiles of one workgroup's (triangle, tile) pairs in an LDS hash table:
+// open addressing over BIN_HASH slots, a bounded probe; a tile that finds no
+// slot goes through the global cursors instead.
+constexpr int BIN_HASH = 2048;
+constexpr int BIN_PROBES = 32;
+constexpr uint32_t BIN_EMPTY = 0xFFFFFFFFu;
+
+__device__ __forceinline__ int bin_slot(uint32_t* keys, uint32_t gt, bool insert) {
+    uint32_t h = (gt * 2654435761u) >> (32 - 11);
+    for (int p = 0; p < BIN_PROBES; p++) {
+        uint32_t sl = (h + (uint32_t)p) & (BIN_HASH - 1);
+        uint32_t k = keys[sl];
+        if (k == gt) return (int)sl;
+        if (k == BIN_EMPTY) {
+            if (!insert) return -1;
+            uint32_t old = atomicCAS(&keys[sl], BIN_EMPTY, gt);
+            if (old == BIN_EMPTY || old == gt) return (int)sl;
+        }
+    }
+    return -1;
+}
+
+// Workgroups below nseg scatter the setup segment of their index: the pairs
+// are counted per tile in LDS, each distinct tile reserves its range of the
+// bin with one global atomic, then every pair takes a place in that range
+// with an LDS atomic.  The other workgroups stride over the clipped fan
+// triangles listed after the segments.
 __global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* passes, Workspace ws, int nseg) {
     if ((int)blockIdx.x < nseg) {
+        __shared__ uint32_t keys[BIN_HASH];
+        __shared__ uint32_t cnt[BIN_HASH];
+        for (int i = threadIdx.x; i < BIN_HASH; i += BLOCK) {
+            keys[i] = BIN_EMPTY;
+            cnt[i] = 0u;
+        }
+        __syncthreads();
         uint32_t n = ws.seg_count[blockIdx.x];
         const uint4* rec = ws.tris + (size_t)blockIdx.x * (BLOCK * FPT);
-        for (uint32_t base = 0; base < n; base += BLOCK) {
-            uint32_t i = base + threadIdx.x;
-            bool have = i < n;
-            bin_record(passes, ws, have, have ? rec[i] : make_uint4(0, 0, 0, 0));
+        for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+            uint4 r = rec[i];
+            const PassDesc& P = passes[r.y];
+            int tx0 = (int)(r.z & 0xFFFFu), ty0 = (int)(r.z >> 16);
+            int tx1 = (int)(r.w & 0xFFFFu), ty1 = (int)(r.w >> 16);
+            for (int ty = ty0; ty <= ty1; ty++)
+                for (int tx = tx0; tx <= tx1; tx++) {
+                    uint32_t gt = (uint32_t)(P.tile0 + ty * P.tiles_x + tx);
+                    int sl = bin_slot(keys, gt, true);
+                    if (sl >= 0) atomicAdd(&cnt[sl], 1u);
+                }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < BIN_HASH; i += BLOCK)
+            if (keys[i] != BIN_EMPTY) cnt[i] = atomicAdd(&ws.tile_cursor[keys[i]], cnt[i]);
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+            uint4 r = rec[i];
+            const PassDesc& P = passes[r.y];
+            int tx0 = (int)(r.z & 0xFFFFu), ty0 = (int)(r.z >> 16);
+            int tx1 = (int)(r.w & 0xFFFFu), ty1 = (int)(r.w >> 16);
+            for (int ty = ty0; ty <= ty1; ty++)
+                for (int tx = tx0; tx <= tx1; tx++) {
+                    uint32_t gt = (uint32_t)(P.tile0 + ty * P.tiles_x + tx);
+                    int sl = bin_slot(keys, gt, false);
+                    uint32_t pos = sl >= 0 ? atomicAdd(&cnt[sl], 1u) : atomicAdd(&ws.tile_cursor[gt], 1u);
+                    if (pos < ws.bin_cap) ws.bins[pos] = r.x;
+                }
         }
         return;
     }
@@ -800,6 +856,7 @@ __device__ void sweep(TileLds& L, const PassDesc& P, const Workspace& ws, uint32
             int npix = wdt * (by1 - by0 + 1);
             if (npix <= SMALL_PIX) {
 #if HANA_EARLY_REJECT
+              if (npix <= HANA_ER_MAX_PIX) {
                 // cheap pass over the candidates, then the exact test on the survivors
                 uint32_t mask = 0;
                 int k = 0;
@@ -813,10 +870,10 @@ __device__ void sweep(TileLds& L, const PassDesc& P, const Workspace& ws, uint32
                     int row = tile_div(b, magic);
                     frag_resolve(L, t, lo, xb, yb, bx0 + b - row * wdt, by0 + row, second);
                 }
-#else
+              } else
+#endif
                 for (int ly = by0; ly <= by1; ly++)
                     for (int lx = bx0; lx <= bx1; lx++) frag_resolve(L, t, lo, xb, yb, lx, ly, second);
-#endif
             } else {
                 L.ax[tid] = t.ax; L.ay[tid] = t.ay; L.s0x[tid] = t.s0x; L.s0y[tid] = t.s0y;
                 L.s1x[tid] = t.s1x; L.s1y[tid] = t.s1y; L.uz[tid] = t.uz;

@@ -28,6 +28,9 @@ constexpr int BLOCK = 256;                 // threads per workgroup (4 wave64)
 #ifndef HANA_EARLY_REJECT
 #define HANA_EARLY_REJECT 1
 #endif
+#ifndef HANA_ER_MAX_PIX
+#define HANA_ER_MAX_PIX 32   // the division-free reject pass runs for small triangles up to this many candidates
+#endif
 constexpr int RBLOCK = HANA_RASTER_BLOCK;  // threads per raster workgroup (one tile)
 static_assert(RBLOCK % 64 == 0 && RBLOCK <= TILE_PIX, "raster workgroup: whole waves, at most one lane per pixel");
 static_assert(HANA_SMALL_PIX >= 1 && HANA_SMALL_PIX <= 32, "small-triangle candidates fit a 32-bit mask");
