@@ -26,7 +26,7 @@ constexpr int BLOCK = 256;                 // threads per workgroup (4 wave64)
 #define HANA_SMALL_PIX 32
 #endif
 #ifndef HANA_EARLY_REJECT
-#define HANA_EARLY_REJECT 1
+#define HANA_EARLY_REJECT 0
 #endif
 #ifndef HANA_ER_MAX_PIX
 #define HANA_ER_MAX_PIX 32   // the division-free reject pass runs for small triangles up to this many candidates
