@@ -959,15 +959,22 @@ __device__ __forceinline__ void shade_pixel(const PassDesc& P, const Workspace& 
     if (P.winner) P.winner[pix] = order;
 }
 
-// Workgroups are dispatched to the 8 XCDs round-robin by blockIdx; the
-// logical tile of a workgroup is chosen so that each XCD receives one
-// contiguous run of tiles (a band of the target), whose vertices, bins and
-// texels then stay in that XCD's L2.
+// Workgroups are dispatched to the 8 XCDs round-robin by blockIdx.  The
+// logical tile of a workgroup is chosen so that each XCD receives whole runs
+// of HANA_XCD_CHUNK consecutive tiles (a piece of a tile row), the runs dealt
+// to the XCDs in turn: vertices, bins and texels of a run stay in one XCD's L2
+// while every XCD still gets tiles from all over the target (objects cover the
+// middle of a frame more than its edges).  Chunk 0 keeps blockIdx order.
+#ifndef HANA_XCD_CHUNK
+#define HANA_XCD_CHUNK 4
+#endif
 __device__ __forceinline__ int xcd_logical_block(int b, int nb) {
-    const int kXcd = 8;
+    const int kXcd = 8, ch = HANA_XCD_CHUNK;
+    if (ch <= 0) return b;
+    int full = nb / (kXcd * ch) * (kXcd * ch);
+    if (b >= full) return b;
     int xcd = b % kXcd, slot = b / kXcd;
-    int per = nb / kXcd, rem = nb % kXcd;
-    return xcd * per + min(xcd, rem) + slot;
+    return ((slot / ch) * kXcd + xcd) * ch + slot % ch;
 }
 
 __global__ __launch_bounds__(RBLOCK) void k_raster(const PassDesc* passes, int npasses, Workspace ws) {
