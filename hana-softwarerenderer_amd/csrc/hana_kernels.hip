@@ -263,128 +263,33 @@ __device__ __forceinline__ float intersect_ratio(const float* p, const float* c,
     }
 }
 
-// Polygon buffers of one lane, kept in LDS: element (vertex v, float k) of
-// the lane at base[(v * NF + k) * CLIP_LANES + lane], so that the lanes of a
-// wave touch consecutive words (no bank conflicts, no scratch memory).
-constexpr int CLIP_LANES = 64;
+// Orders the LDS accesses of the lanes of one wave (the wave executes them in
+// order; this keeps the compiler from moving them across).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A queued face is clipped by a group of CLIP_GROUP lanes of one wave, lane i
+// of the group owning vertex i of the polygon.  Its two polygon buffers
+// (clip position then POLY_VARY varyings per vertex) live in LDS.
+constexpr int CLIP_GROUP = 16;
+constexpr int CLIP_THREADS = 1024;
+constexpr int CLIP_FACES = CLIP_THREADS / CLIP_GROUP;   // faces per workgroup and round
 constexpr int CLIP_NF = 4 + POLY_VARY;
-template <int NF>
-struct LanePoly {
-    float* base;
-    int lane;
-    __device__ __forceinline__ float& at(int v, int k) const { return base[(v * NF + k) * CLIP_LANES + lane]; }
-};
+static_assert(POLY_MAX <= CLIP_GROUP, "one lane per polygon vertex");
 
-// One Sutherland-Hodgman stage over vertices of NF floats (clip pos first);
-// every float is interpolated as liner_interpolate_varyings does
-// (SR/graphics.cpp:7-17, 73-110).  A convex polygon gains at most one vertex
-// per plane (3 + 7 = POLY_MAX); an output that would exceed POLY_MAX (only
-// possible through rounding) returns -1 and the polygon is dropped and flagged.
-template <int NF>
-__device__ __forceinline__ int clip_stage(int plane, int n, LanePoly<NF> in, LanePoly<NF> out) {
-    int m = 0;
-    for (int i = 0; i < n; i++) {
-        int prev = (i - 1 + n) % n;
-        float pc[4], cc[4];
-        for (int k = 0; k < 4; k++) { pc[k] = in.at(prev, k); cc[k] = in.at(i, k); }
-        bool pin = inside_plane(pc, plane), cin = inside_plane(cc, plane);
-        if (m + (pin != cin) + (cin ? 1 : 0) > POLY_MAX) return -1;
-        if (pin != cin) {
-            float t = intersect_ratio(pc, cc, plane);
-            for (int k = 0; k < NF; k++) {
-                float f = in.at(prev, k);
-                out.at(m, k) = f + (in.at(i, k) - f) * t;
-            }
-            m++;
-        }
-        if (cin) {
-            for (int k = 0; k < NF; k++) out.at(m, k) = in.at(i, k);
-            m++;
-        }
-    }
-    return m;
-}
-// True when every vertex of the polygon is inside the plane: clip_stage would
-// then return the same vertices in the same order, so the stage is skipped.
-template <int NF>
-__device__ __forceinline__ bool all_inside(int plane, int n, LanePoly<NF> p) {
-    for (int i = 0; i < n; i++) {
-        float c[4];
-        for (int k = 0; k < 4; k++) c[k] = p.at(i, k);
-        if (!inside_plane(c, plane)) return false;
-    }
-    return true;
-}
-
-// The 7 planes in order on the triangle in `cur`.  Returns the vertex count
-// with the result in `cur` (the buffers swap per stage), 0 when fewer than 3
-// remain, -1 for a polygon beyond POLY_MAX vertices.
-template <int NF>
-__device__ __forceinline__ int clip_polygon(LanePoly<NF>& cur, LanePoly<NF>& other) {
-    int n = 3;
-    for (int plane = PL_W; plane <= PL_NZ; plane++) {
-        if (all_inside<NF>(plane, n, cur)) continue;
-        n = clip_stage<NF>(plane, n, cur, other);
-        if (n < 3) return n < 0 ? -1 : 0;
-        LanePoly<NF> t = cur;
-        cur = other;
-        other = t;
-    }
-    return n;
-}
-
-// Re-runs the vertex program on the face's corners and clips the triangle in
-// the lane's LDS buffers.  Returns the vertex count (>= 3) with the polygon in
-// buffer b when *in_b, else in a; 0 when nothing remains.
-// lds: 2 * POLY_MAX * CLIP_NF * CLIP_LANES floats shared by the wave.
-__device__ __noinline__ int clip_face(const PassDesc& P, const Workspace& ws, int f, float* lds, int lane, int* in_b) {
-    const DevUniforms& u = *P.u;
-    LanePoly<CLIP_NF> a{lds, lane};
-    LanePoly<CLIP_NF> b{lds + POLY_MAX * CLIP_NF * CLIP_LANES, lane};
-    bool need_n = P.kind != SHADER_SHADOW && P.kind != SHADER_TEXTURE;
-    for (int k = 0; k < 3; k++) {
-        int v = P.idx[3 * f + k];
-        F3 p = f3(P.px[v], P.py[v], P.pz[v]);
-        F3 n = need_n ? f3(P.nx[v], P.ny[v], P.nz[v]) : f3(0.f, 0.f, 0.f);
-        float clip[4], vary[8];
-        for (int c = 0; c < 8; c++) vary[c] = 0.0f;
-        vertex_program(P.kind, u, p, n, P.tu[v], P.tv[v], clip, vary);
-        for (int c = 0; c < 4; c++) a.at(k, c) = clip[c];
-        for (int c = 0; c < POLY_VARY; c++) a.at(k, 4 + c) = vary[c];
-    }
-    // positions only first (in b's space): faces that vanish are dropped
-    // without interpolating varyings
-    {
-        LanePoly<4> pa{b.base, lane};
-        LanePoly<4> pb{b.base + POLY_MAX * 4 * CLIP_LANES, lane};
-        for (int k = 0; k < 3; k++)
-            for (int c = 0; c < 4; c++) pa.at(k, c) = a.at(k, c);
-        int np = clip_polygon<4>(pa, pb);
-        if (np < 0) atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_POLY_TOO_LARGE);
-        if (np < 3) return 0;
-    }
-    LanePoly<CLIP_NF> cur = a, oth = b;
-    int n = clip_polygon<CLIP_NF>(cur, oth);
-    if (n < 3) return 0;
-    *in_b = cur.base == b.base;
-    return n;
-}
-
-// Stores a clipped polygon (perspective-divided) in the pool slot pid.
-__device__ void store_polygon(const Workspace& ws, LanePoly<CLIP_NF> cur, int n, int pid, int pass, int f) {
-    for (int k = 0; k < n; k++) {
-        float w = cur.at(k, 3);
-        float4 r;
-        r.w = 1 / w;
-        r.z = cur.at(k, 2) / w;
-        r.y = cur.at(k, 1) / w;
-        r.x = cur.at(k, 0) / w;
-        ws.pool_vr[(size_t)pid * POLY_MAX + k] = r;
-        for (int c = 0; c < POLY_VARY; c++) ws.pool_vs[((size_t)pid * POLY_MAX + k) * POLY_VARY + c] = cur.at(k, 4 + c);
-    }
-    ws.pool_face[pid] = f;
-    ws.pool_pass[pid] = pass;
-    ws.pool_n[pid] = n;
+// The homogeneous divide of a clip-space vertex, as the face loop applies it
+// to the clipped vertices (SR/graphics.cpp:317, 336).
+__device__ __forceinline__ float4 ndc_of(const float* v) {
+    float w = v[3];
+    float4 r;
+    r.w = 1 / w;
+    r.z = v[2] / w;
+    r.y = v[1] / w;
+    r.x = v[0] / w;
+    return r;
 }
 
 // ---------------------------------------------------------------------------
@@ -560,50 +465,132 @@ __global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* passes, int npa
     }
 }
 
-// Clips the queued faces: one wave per workgroup, grid-stride over the queue,
-// polygon buffers in LDS; pool slots and the fan triangles' records (0, j+1,
-// j+2; after the setup segments of ws.tris) are reserved once per wave.
-__global__ __launch_bounds__(CLIP_LANES) void k_clip(const PassDesc* passes, Workspace ws) {
-    __shared__ float lds[2 * POLY_MAX * CLIP_NF * CLIP_LANES];
+// Clips the queued faces, CLIP_FACES per workgroup and round, grid-stride
+// over the queue.  Per group of CLIP_GROUP lanes: the vertex program runs on
+// the three corners in three lanes (the same program as k_vertex, so the
+// same bits); then each of the seven planes of clip_triangle
+// (SR/graphics.cpp:136-161) in order, one Sutherland-Hodgman stage
+// (SR/graphics.cpp:73-110) computed vertex-parallel: lane i looks at the edge
+// (i-1, i) and emits the intersection when the two ends differ in side and
+// vertex i when it is inside, at the place the prefix sum of the lanes'
+// emissions gives it -- the order and the arithmetic of the sequential loop.
+// A stage with every vertex inside would copy the polygon unchanged and is
+// skipped.  Pool slots (one per polygon) and the fan triangles' records
+// (after the setup segments of ws.tris) are reserved once per workgroup and
+// round.
+__global__ __launch_bounds__(CLIP_THREADS) void k_clip(const PassDesc* passes, Workspace ws) {
+    __shared__ float buf[CLIP_FACES][2][POLY_MAX][CLIP_NF];
+    __shared__ uint32_t s_npoly, s_ntri, s_poly0, s_tri0;
     uint32_t count = min(ws.status[STATUS_CLIPQ], ws.clipq_cap);
     uint32_t clip_cap = ws.tris_cap - ws.seg_region;
-    for (uint32_t base = blockIdx.x * CLIP_LANES; base < count; base += gridDim.x * CLIP_LANES) {
-        uint32_t q = base + threadIdx.x;
-        bool have = q < count;
-        int pass = 0, f = 0, n = 0, in_b = 0;
+    const int tid = threadIdx.x, lane = tid & 63, g = tid / CLIP_GROUP, li = tid % CLIP_GROUP;
+    const uint64_t gmask = 0xFFFFull << (lane & ~(CLIP_GROUP - 1));
+    for (uint32_t base = blockIdx.x * CLIP_FACES; base < count; base += gridDim.x * CLIP_FACES) {
+        if (tid == 0) { s_npoly = 0; s_ntri = 0; }
+        uint32_t q = base + (uint32_t)g;
+        bool have = q < count;                       // uniform within the group
+        int pass = 0, f = 0;
         if (have) {
             uint64_t e = ws.clip_queue[q];
             pass = (int)(e >> 32);
             f = (int)(uint32_t)e;
-            n = clip_face(passes[pass], ws, f, lds, threadIdx.x, &in_b);
         }
-        bool keep = n >= 3;
-        uint32_t pid = wave_reserve(&ws.status[STATUS_POLYS], 0, keep);
+        const PassDesc& P = passes[pass];
+        if (have && li < 3) {
+            int v = P.idx[3 * f + li];
+            bool need_n = P.kind != SHADER_SHADOW && P.kind != SHADER_TEXTURE;
+            F3 p = f3(P.px[v], P.py[v], P.pz[v]);
+            F3 nn = need_n ? f3(P.nx[v], P.ny[v], P.nz[v]) : f3(0.f, 0.f, 0.f);
+            float clip[4], vary[8];
+            for (int c = 0; c < 8; c++) vary[c] = 0.0f;
+            vertex_program(P.kind, *P.u, p, nn, P.tu[v], P.tv[v], clip, vary);
+            for (int c = 0; c < 4; c++) buf[g][0][li][c] = clip[c];
+            for (int c = 0; c < POLY_VARY; c++) buf[g][0][li][4 + c] = vary[c];
+        }
+        wave_sync();
+        int n = have ? 3 : 0, cur = 0;
+        bool too_large = false;
+        for (int plane = PL_W; plane <= PL_NZ; plane++) {
+            bool act = li < n;
+            bool cin = act && inside_plane(buf[g][cur][li], plane);
+            uint64_t am = __ballot(act) & gmask, im = __ballot(cin) & gmask;
+            if (n < 3 || am == im) continue;         // dead, or every vertex inside
+            int prev = (li - 1 + n) % n;
+            bool pin = act && inside_plane(buf[g][cur][prev], plane);
+            int emit = act ? (pin != cin) + (cin ? 1 : 0) : 0;
+            int incl = emit;
+            for (int d = 1; d < CLIP_GROUP; d <<= 1) {
+                int o = __shfl_up(incl, d, CLIP_GROUP);
+                if (li >= d) incl += o;
+            }
+            int m = __shfl(incl, CLIP_GROUP - 1, CLIP_GROUP);
+            if (m > POLY_MAX) {                      // only possible through rounding
+                too_large = true;
+                n = 0;
+                continue;
+            }
+            int o = incl - emit;
+            float(*in)[CLIP_NF] = buf[g][cur];
+            float(*out)[CLIP_NF] = buf[g][cur ^ 1];
+            if (act && pin != cin) {
+                float t = intersect_ratio(in[prev], in[li], plane);
+                for (int k = 0; k < CLIP_NF; k++) {
+                    float fv = in[prev][k];
+                    out[o][k] = fv + (in[li][k] - fv) * t;
+                }
+                o++;
+            }
+            if (cin)
+                for (int k = 0; k < CLIP_NF; k++) out[o][k] = in[li][k];
+            n = m < 3 ? 0 : m;
+            cur ^= 1;
+            wave_sync();
+        }
+        if (too_large && li == 0) atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_POLY_TOO_LARGE);
+        float(*poly)[CLIP_NF] = buf[g][cur];
+        bool keep = n >= 3;                          // uniform within the group
+        // fan triangle (0, li+1, li+2) of the polygon in lane li
+        TriS t;
+        bool ok = keep && li < n - 2 && tri_setup(ndc_of(poly[0]), ndc_of(poly[li + 1]), ndc_of(poly[li + 2]), P.width,
+                                                P.height, t);
+        __syncthreads();
+        uint32_t lpoly = wave_reserve_lds(&s_npoly, keep && li == 0);
+        uint32_t ltri = wave_reserve_lds(&s_ntri, ok);
+        __syncthreads();
+        if (tid == 0) {
+            s_poly0 = s_npoly ? atomicAdd(&ws.status[STATUS_POLYS], s_npoly) : 0u;
+            s_tri0 = s_ntri ? atomicAdd(&ws.status[STATUS_TRIS], s_ntri) : 0u;
+        }
+        __syncthreads();
+        uint32_t pid = __shfl(s_poly0 + lpoly, lane & ~(CLIP_GROUP - 1));
+        bool reserved = ok;
         if (keep && pid >= ws.poly_cap) {
-            atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_POLY_OVERFLOW);
+            if (li == 0) atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_POLY_OVERFLOW);
             keep = false;
+            ok = false;
         }
-        if (keep) {
-            LanePoly<CLIP_NF> cur{lds + (in_b ? POLY_MAX * CLIP_NF * CLIP_LANES : 0), (int)threadIdx.x};
-            store_polygon(ws, cur, n, (int)pid, pass, f);
-            stat_add(passes[pass], ST_POLYS, 1);
+        uint32_t slot = s_tri0 + ltri;
+        if (reserved && slot >= clip_cap) {
+            atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_TRIS_OVERFLOW);
+            ok = reserved = false;
         }
-        if (have) passes[pass].poly_of_face[f] = keep ? (int)pid : -1;
-        int nt = keep ? n - 2 : 0;
-        for (int j = 0; __ballot(j < nt); j++) {
-            TriS t;
-            bool ok = false;
-            if (j < nt) {
-                const float4* pv = ws.pool_vr + (size_t)pid * POLY_MAX;
-                ok = tri_setup(pv[0], pv[j + 1], pv[j + 2], passes[pass].width, passes[pass].height, t);
+        // a reserved record of a dropped polygon is written empty (no tiles) for k_bin
+        if (reserved && !ok) ws.tris[ws.seg_region + slot] = make_uint4(0u, 0u, 1u, 0u);
+        if (keep && li < n) {
+            ws.pool_vr[(size_t)pid * POLY_MAX + li] = ndc_of(poly[li]);
+            for (int c = 0; c < POLY_VARY; c++) ws.pool_vs[((size_t)pid * POLY_MAX + li) * POLY_VARY + c] = poly[li][4 + c];
+        }
+        if (have && li == 0) {
+            if (keep) {
+                ws.pool_face[pid] = f;
+                ws.pool_pass[pid] = pass;
+                ws.pool_n[pid] = n;
+                stat_add(P, ST_POLYS, 1);
             }
-            uint32_t slot = wave_reserve(&ws.status[STATUS_TRIS], 0, ok);
-            if (ok && slot >= clip_cap) {
-                atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_TRIS_OVERFLOW);
-                ok = false;
-            }
-            count_tiles(passes[pass], ws, pass, ok, t, CLIPPED_BIT | (pid << 3) | (uint32_t)j, ws.seg_region + slot);
+            P.poly_of_face[f] = keep ? (int)pid : -1;
         }
+        count_tiles(P, ws, pass, ok, t, CLIPPED_BIT | (pid << 3) | (uint32_t)li, ws.seg_region + slot);
+        __syncthreads();
     }
 }
 
@@ -1070,7 +1057,7 @@ hipError_t launch_setup(const PassDesc* d_passes, int npasses, const Workspace& 
 }
 hipError_t launch_clip(const PassDesc* d_passes, const Workspace& ws, hipStream_t s) {
     // grid-stride: the queue length is only known on the device
-    hipLaunchKernelGGL(k_clip, dim3(2048), dim3(CLIP_LANES), 0, s, d_passes, ws);
+    hipLaunchKernelGGL(k_clip, dim3(1024), dim3(CLIP_THREADS), 0, s, d_passes, ws);
     return hipGetLastError();
 }
 int scan_chunks(int ntiles) { return ntiles > 0 ? (ntiles + SCAN_CHUNK - 1) / SCAN_CHUNK : 1; }
