@@ -34,8 +34,14 @@ constexpr int BLOCK = 256;                 // threads per workgroup (4 wave64)
 constexpr int RBLOCK = HANA_RASTER_BLOCK;  // threads per raster workgroup (one tile)
 static_assert(RBLOCK % 64 == 0 && RBLOCK <= TILE_PIX, "raster workgroup: whole waves, at most one lane per pixel");
 static_assert(HANA_SMALL_PIX >= 1 && HANA_SMALL_PIX <= 32, "small-triangle candidates fit a 32-bit mask");
-constexpr int VPT = 4;                     // vertices per thread in k_vertex
-constexpr int FPT = 4;                     // faces per thread in k_setup
+#ifndef HANA_VPT
+#define HANA_VPT 4
+#endif
+#ifndef HANA_FPT
+#define HANA_FPT 4
+#endif
+constexpr int VPT = HANA_VPT;              // vertices per thread in k_vertex
+constexpr int FPT = HANA_FPT;              // faces per thread in k_setup
 constexpr int POLY_MAX = 10;               // vertices of a clipped polygon (3 + 7 planes)
 constexpr int POLY_VARY = 8;               // varyings stored per clipped vertex
 constexpr uint32_t NO_WINNER = 0xFFFFFFFFu;

@@ -168,3 +168,56 @@ def test_config5_full(ctx, oracle):
     scene = sg.config5_scene()
     rep, st = check_scene(ctx, oracle, scene)
     assert st["faces"] == 19_998_784
+
+
+def straddling_scene(n=40_000):
+    """n copies of one triangle that crosses the left frustum plane: every
+    face is clipped into a polygon (two fan triangles each), far more than
+    the polygon pool and the fan-triangle list hold at first, and all of
+    them tie in depth (the last copy wins every pixel)."""
+    tri = np.array([[-3.0, -0.4, 0.0], [0.5, -0.3, 0.0], [0.2, 0.5, 0.0]], np.float32)
+    uv = np.array([[0.1, 0.2], [0.8, 0.3], [0.5, 0.9]], np.float32)
+    mesh = sg.Mesh(np.tile(tri, (n, 1)), np.tile(np.array([[0, 0, 1]], np.float32), (3 * n, 1)),
+                   np.tile(uv, (n, 1)), np.arange(3 * n, dtype=np.int32).reshape(n, 3))
+    d, nm = sg.texture_pair(32, 32, 5)
+    return sg.Scene(mesh, 48, 32, sg.BLINN, False, (0.0, 0.0, 2.0), (2.0, 2.0, 2.0), d, nm, name="straddling")
+
+
+def test_clip_pool_overflow_regrows(ctx, oracle):
+    """The first render overflows the clip pool and reports HANA_E_OVERFLOW at
+    the read; the library has grown its buffers and the re-render is exact."""
+    from hana_amd.gpu import HANA_E_OVERFLOW, HanaError, build_uniforms
+    scene = straddling_scene()
+    u = build_uniforms(scene)
+    mesh, dif, nrm = ctx.mesh(scene.mesh), ctx.texture(scene.diffuse), ctx.texture(scene.normal)
+    tgt = ctx.target(scene.width, scene.height, winner=True)
+    ctx.clear(tgt)
+    ctx.draw(mesh, scene.shader, u, tgt, dif, nrm)
+    with pytest.raises(HanaError) as err:
+        ctx.read(tgt, scene.width, scene.height)
+    assert err.value.code == HANA_E_OVERFLOW
+    ctx.clear(tgt)
+    ctx.draw(mesh, scene.shader, u, tgt, dif, nrm)
+    col, dep = ctx.read(tgt, scene.width, scene.height)
+    win = ctx.read_winner(tgt, scene.width, scene.height)
+    ref, owin, _, st, _ = oracle_passes(oracle, scene)
+    assert st["clipped_faces"] == 40_000
+    assert np.array_equal(win.astype(np.int64), as_u32_ids(owin))
+    assert_depth_exact(dep, ref.depth, "straddling")
+    assert_color_ok(col, ref.color, "straddling")
+
+
+def test_clip_pool_overflow_in_a_batch(ctx, oracle):
+    """Batch finalisation sizes the pool and the lists with its trial run."""
+    from hana_amd.gpu import Batch, SceneOnDevice
+    scene = straddling_scene()
+    dev = SceneOnDevice(ctx, scene)
+    batch = Batch(ctx)
+    batch.add(dev.desc())
+    batch.finalize(use_graph=True)
+    batch.run()
+    col, dep = dev.read()
+    tgt, _, _ = oracle.render_frame(scene)
+    assert_depth_exact(dep, tgt.depth, "straddling batch")
+    assert_color_ok(col, tgt.color, "straddling batch")
+    batch.close()
