@@ -183,9 +183,16 @@ def straddling_scene(n=40_000):
     return sg.Scene(mesh, 48, 32, sg.BLINN, False, (0.0, 0.0, 2.0), (2.0, 2.0, 2.0), d, nm, name="straddling")
 
 
-def test_clip_pool_overflow_regrows(ctx, oracle):
+def test_clip_pool_overflow_regrows(oracle):
     """The first render overflows the clip pool and reports HANA_E_OVERFLOW at
-    the read; the library has grown its buffers and the re-render is exact."""
+    the read; the library has grown its buffers and the re-render is exact.
+    A context of its own, so that no earlier test has grown the pool."""
+    from hana_amd.gpu import Context
+    with Context(0) as ctx:
+        check_regrowth(ctx, oracle)
+
+
+def check_regrowth(ctx, oracle):
     from hana_amd.gpu import HANA_E_OVERFLOW, HanaError, build_uniforms
     scene = straddling_scene()
     u = build_uniforms(scene)
