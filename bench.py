@@ -116,6 +116,22 @@ def bench_batched(ctx, args):
     t_batch = (time.perf_counter() - t0) / reps
     same = all(np.array_equal(d.read()[0], c) for d, c in zip(devs[:8], per_frame_out))
     batch.close()
+    # per-stage kernel times of the same launch sequence, run eagerly with events
+    eager = Batch(ctx)
+    for d in devs:
+        eager.add(d.desc())
+    eager.finalize(use_graph=False)
+    eager.run()
+    ctx.sync()
+    ctx.read_timing()
+    ctx.set_timing(True)
+    for _ in range(reps):
+        eager.run()
+    ctx.sync()
+    st = ctx.read_timing()
+    ctx.set_timing(False)
+    eager.close()
+    stage_ms = {k: round(v[0] / v[1], 4) if v[1] else 0.0 for k, v in st.items()}
     out = {"workload": f"{len(scenes)} scenes, {sum(s.mesh.nfaces for s in scenes)} tris, "
                        f"{sum(s.width * s.height for s in scenes)} px, {sum(s.shadow for s in scenes)} with shadow",
            "per_frame_ms_per_batch": round(t_frame * 1e3, 3), "batched_ms_per_batch": round(t_batch * 1e3, 3),
@@ -124,7 +140,7 @@ def bench_batched(ctx, args):
            "speedup": round(t_frame / t_batch, 2), "fragments_per_batch": frags,
            "per_frame_Mfrag_s": round(frags / t_frame / 1e6, 2) if frags else None,
            "batched_Mfrag_s": round(frags / t_batch / 1e6, 2) if frags else None,
-           "batched_equals_per_frame": same, "cpu_baseline": cpu}
+           "batched_equals_per_frame": same, "stage_ms": stage_ms, "cpu_baseline": cpu}
     log("batched:", json.dumps(out))
     return out
 

@@ -339,6 +339,10 @@ __device__ __forceinline__ void wave_count(uint32_t* counters, int t, bool want)
     }
 }
 
+// Triangles over more tiles than this are counted and binned by a whole wave
+// (or workgroup), their tiles spread over the lanes.
+constexpr int WIDE_TILES = 16;
+
 // Counts every (triangle, tile) pair of one lane's triangle and writes its
 // record (bin entry, pass, first and last tile) for k_bin to ws.tris[slot];
 // wave-uniform.
@@ -355,9 +359,21 @@ __device__ __forceinline__ void count_tiles(const PassDesc& P, const Workspace& 
         ws.tris[slot] = make_uint4(entry, (uint32_t)pass, (uint32_t)tx0 | ((uint32_t)ty0 << 16),
                                    (uint32_t)tx1 | ((uint32_t)ty1 << 16));
     }
-    for (int k = 0; __ballot(k < n); k++) {
-        bool want = k < n;
-        int gt = want ? P.tile0 + (ty0 + k / nx) * P.tiles_x + tx0 + k % nx : 0;
+    // a triangle over more than WIDE_TILES tiles is counted by the whole wave
+    int tile0 = P.tile0, tiles_x = P.tiles_x;
+    uint64_t wide = __ballot(n > WIDE_TILES);
+    while (wide) {
+        int src = __ffsll((unsigned long long)wide) - 1;
+        wide &= wide - 1;
+        int wn = __shfl(n, src), wnx = __shfl(nx, src), wx0 = __shfl(tx0, src), wy0 = __shfl(ty0, src);
+        int wt0 = __shfl(tile0, src), wtx = __shfl(tiles_x, src);
+        for (int k = __lane_id(); k < wn; k += 64)
+            atomicAdd(&ws.tile_count[wt0 + (wy0 + k / wnx) * wtx + wx0 + k % wnx], 1u);
+    }
+    int ns = n > WIDE_TILES ? 0 : n;
+    for (int k = 0; __ballot(k < ns); k++) {
+        bool want = k < ns;
+        int gt = want ? tile0 + (ty0 + k / nx) * tiles_x + tx0 + k % nx : 0;
         wave_count(ws.tile_count, gt, want);
     }
     if (P.stats) {
@@ -606,8 +622,21 @@ __device__ __forceinline__ void bin_record(const PassDesc* passes, const Workspa
         tile0 = P.tile0;
         tiles_x = P.tiles_x;
     }
-    for (int k = 0; __ballot(k < n); k++) {
-        bool want = k < n;
+    uint64_t wide = __ballot(n > WIDE_TILES);
+    while (wide) {
+        int src = __ffsll((unsigned long long)wide) - 1;
+        wide &= wide - 1;
+        int wn = __shfl(n, src), wnx = __shfl(nx, src), wx0 = __shfl(tx0, src), wy0 = __shfl(ty0, src);
+        int wt0 = __shfl(tile0, src), wtx = __shfl(tiles_x, src);
+        uint32_t we = __shfl(r.x, src);
+        for (int k = __lane_id(); k < wn; k += 64) {
+            uint32_t pos = atomicAdd(&ws.tile_cursor[wt0 + (wy0 + k / wnx) * wtx + wx0 + k % wnx], 1u);
+            if (pos < ws.bin_cap) ws.bins[pos] = we;
+        }
+    }
+    int ns = n > WIDE_TILES ? 0 : n;
+    for (int k = 0; __ballot(k < ns); k++) {
+        bool want = k < ns;
         int gt = want ? tile0 + (ty0 + k / nx) * tiles_x + tx0 + k % nx : 0;
         uint32_t pos = wave_reserve(ws.tile_cursor, gt, want);
         if (want && pos < ws.bin_cap) ws.bins[pos] = r.x;
@@ -645,10 +674,13 @@ __global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* passes, Workspace
     if ((int)blockIdx.x < nseg) {
         __shared__ uint32_t keys[BIN_HASH];
         __shared__ uint32_t cnt[BIN_HASH];
+        __shared__ uint16_t wide[BLOCK * FPT];
+        __shared__ uint32_t nwide;
         for (int i = threadIdx.x; i < BIN_HASH; i += BLOCK) {
             keys[i] = BIN_EMPTY;
             cnt[i] = 0u;
         }
+        if (threadIdx.x == 0) nwide = 0;
         __syncthreads();
         uint32_t n = ws.seg_count[blockIdx.x];
         const uint4* rec = ws.tris + (size_t)blockIdx.x * (BLOCK * FPT);
@@ -657,6 +689,10 @@ __global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* passes, Workspace
             const PassDesc& P = passes[r.y];
             int tx0 = (int)(r.z & 0xFFFFu), ty0 = (int)(r.z >> 16);
             int tx1 = (int)(r.w & 0xFFFFu), ty1 = (int)(r.w >> 16);
+            if ((tx1 - tx0 + 1) * (ty1 - ty0 + 1) > WIDE_TILES) {   // binned by the workgroup below
+                wide[atomicAdd(&nwide, 1u)] = (uint16_t)i;
+                continue;
+            }
             for (int ty = ty0; ty <= ty1; ty++)
                 for (int tx = tx0; tx <= tx1; tx++) {
                     uint32_t gt = (uint32_t)(P.tile0 + ty * P.tiles_x + tx);
@@ -673,6 +709,7 @@ __global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* passes, Workspace
             const PassDesc& P = passes[r.y];
             int tx0 = (int)(r.z & 0xFFFFu), ty0 = (int)(r.z >> 16);
             int tx1 = (int)(r.w & 0xFFFFu), ty1 = (int)(r.w >> 16);
+            if ((tx1 - tx0 + 1) * (ty1 - ty0 + 1) > WIDE_TILES) continue;
             for (int ty = ty0; ty <= ty1; ty++)
                 for (int tx = tx0; tx <= tx1; tx++) {
                     uint32_t gt = (uint32_t)(P.tile0 + ty * P.tiles_x + tx);
@@ -680,6 +717,19 @@ __global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* passes, Workspace
                     uint32_t pos = sl >= 0 ? atomicAdd(&cnt[sl], 1u) : atomicAdd(&ws.tile_cursor[gt], 1u);
                     if (pos < ws.bin_cap) ws.bins[pos] = r.x;
                 }
+        }
+        // wide triangles, one at a time, their tiles spread over the workgroup
+        uint32_t nw = nwide;
+        for (uint32_t w = 0; w < nw; w++) {
+            uint4 r = rec[wide[w]];
+            const PassDesc& P = passes[r.y];
+            int tx0 = (int)(r.z & 0xFFFFu), ty0 = (int)(r.z >> 16);
+            int nx = (int)(r.w & 0xFFFFu) - tx0 + 1;
+            int nt = nx * ((int)(r.w >> 16) - ty0 + 1);
+            for (int k = threadIdx.x; k < nt; k += BLOCK) {
+                uint32_t pos = atomicAdd(&ws.tile_cursor[P.tile0 + (ty0 + k / nx) * P.tiles_x + tx0 + k % nx], 1u);
+                if (pos < ws.bin_cap) ws.bins[pos] = r.x;
+            }
         }
         return;
     }
