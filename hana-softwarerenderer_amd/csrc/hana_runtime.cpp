@@ -443,20 +443,19 @@ int hana_gpu_create(int device, hana_ctx** out) {
     if (device < 0 || device >= count) return HANA_E_INVALID;
     hana_ctx* c = new hana_ctx();
     c->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    auto undo = [c]() {
+        for (int i = 0; i < kRing; i++)
+            if (c->ring_event[i]) (void)hipEventDestroy(c->ring_event[i]);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        (void)hipGetLastError();
         delete c;
         return HANA_E_DEVICE;
-    }
-    for (int i = 0; i < kRing; i++) {
-        if (hipEventCreateWithFlags(&c->ring_event[i], hipEventDisableTiming) != hipSuccess) {
-            delete c;
-            return HANA_E_DEVICE;
-        }
-    }
-    if (hipHostMalloc((void**)&c->status_host, sizeof(uint32_t) * STATUS_WORDS) != hipSuccess) {
-        delete c;
-        return HANA_E_DEVICE;
-    }
+    };
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return undo();
+    for (int i = 0; i < kRing; i++)
+        if (hipEventCreateWithFlags(&c->ring_event[i], hipEventDisableTiming) != hipSuccess) return undo();
+    if (hipHostMalloc((void**)&c->status_host, sizeof(uint32_t) * STATUS_WORDS) != hipSuccess) return undo();
     std::memset(c->status_host, 0, sizeof(uint32_t) * STATUS_WORDS);
     *out = c;
     return HANA_OK;
@@ -577,8 +576,14 @@ int hana_mesh_create(hana_ctx* c, int32_t nverts, const float* pos, const float*
         delete m;
         return fail(c, HANA_E_NOMEM, "hana_mesh_create: device allocation failed");
     }
-    HIP_TRY(c, hipMemcpy(m->soa, soa.data(), sizeof(float) * soa.size(), hipMemcpyHostToDevice));
-    if (nfaces) HIP_TRY(c, hipMemcpy(m->idx, idx, sizeof(int32_t) * 3 * (size_t)nfaces, hipMemcpyHostToDevice));
+    hipError_t e = hipMemcpy(m->soa, soa.data(), sizeof(float) * soa.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && nfaces) e = hipMemcpy(m->idx, idx, sizeof(int32_t) * 3 * (size_t)nfaces, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(m->soa);
+        (void)hipFree(m->idx);
+        delete m;
+        return hip_fail(c, e, "hana_mesh_create: upload");
+    }
     *out = m;
     return HANA_OK;
 }
@@ -610,7 +615,12 @@ int hana_texture_create(hana_ctx* c, int32_t w, int32_t h, int32_t bpp, const ui
         delete t;
         return fail(c, HANA_E_NOMEM, "hana_texture_create: device allocation failed");
     }
-    HIP_TRY(c, hipMemcpy(t->texels, words.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    hipError_t e = hipMemcpy(t->texels, words.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(t->texels);
+        delete t;
+        return hip_fail(c, e, "hana_texture_create: upload");
+    }
     *out = t;
     return HANA_OK;
 }
