@@ -192,7 +192,7 @@ __global__ __launch_bounds__(BLOCK) void k_vertex(const PassDesc* passes, int np
     int gtid = blockIdx.x * BLOCK + threadIdx.x;
     int gsize = gridDim.x * BLOCK;
     for (int i = gtid; i < ws.ntiles; i += gsize) ws.tile_count[i] = 0;
-    if (gtid < STATUS_WORDS) ws.status[gtid] = 0;
+    if (gtid < STATUS_SEQ_WORDS) ws.status[gtid] = 0;
 
     const PassDesc& P = passes[find_pass<0>(passes, npasses, blockIdx.x)];
     const DevUniforms& u = *P.u;
@@ -768,6 +768,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(Workspace ws) {
         if (b + i < n) s += ws.tile_count[b + i];
     uint32_t t = block_sum(s, sh);
     if (threadIdx.x == 0) ws.scan_part[blockIdx.x] = t;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {   // set-up and clipping are complete
+        atomicMax(&ws.status[STATUS_NEED_POLYS], ws.status[STATUS_POLYS]);
+        atomicMax(&ws.status[STATUS_NEED_TRIS], ws.status[STATUS_TRIS]);
+    }
 }
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(Workspace ws) {
@@ -807,6 +811,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(Workspace ws) {
         // `run` of the last thread is the grand total
         ws.tile_offset[n] = run;
         ws.status[STATUS_BINS_NEEDED] = run;
+        atomicMax(&ws.status[STATUS_NEED_BINS], run);
         if (run > ws.bin_cap) atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_BIN_OVERFLOW);
     }
 }

@@ -201,7 +201,9 @@ int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_p
         if ((r = ensure(c, A.clip_queue, sizeof(uint64_t) * want_q))) return r;
         A.clipq_cap = (uint32_t)want_q;
     }
+    bool fresh_status = A.status.p == nullptr;
     if ((r = ensure(c, A.status, sizeof(uint32_t) * STATUS_WORDS))) return r;
+    if (fresh_status) HIP_TRY(c, launch_fill32(A.status.as<uint32_t>(), 0u, STATUS_WORDS, c->stream));
     if ((r = ensure(c, A.stats, sizeof(uint64_t) * ST_WORDS * std::max(n, 1)))) return r;
     for (int i = 0; i < n; i++) {
         PassDesc& d = pl.descs[i];
@@ -345,50 +347,56 @@ int upload_tables(hana_ctx* c, const Plan& pl, Arena& A) {
     return HANA_OK;
 }
 
+// Reads the sticky status of every sequence since the last check (waits for
+// the stream).  On an overflow the buffers are grown to the largest need
+// seen, the sticky words are cleared, and HANA_E_OVERFLOW asks the caller to
+// render again.
 int check_status(hana_ctx* c) {
     if (!c->status_pending) return HANA_OK;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     c->status_pending = false;
     uint32_t flags = c->status_host[STATUS_FLAGS];
+    if (!flags) return HANA_OK;
+    uint32_t need_bins = c->status_host[STATUS_NEED_BINS];
+    uint32_t need_polys = c->status_host[STATUS_NEED_POLYS];
+    uint32_t need_tris = c->status_host[STATUS_NEED_TRIS];
+    HIP_TRY(c, launch_fill32(c->arena.status.as<uint32_t>() + STATUS_SEQ_WORDS, 0u, STATUS_WORDS - STATUS_SEQ_WORDS,
+                             c->stream));
+    std::memset(c->status_host + STATUS_SEQ_WORDS, 0, sizeof(uint32_t) * (STATUS_WORDS - STATUS_SEQ_WORDS));
     if (flags & FLAG_POLY_TOO_LARGE)
         return fail(c, HANA_E_OVERFLOW, "a clipped polygon exceeded 10 vertices and was dropped");
-    if (flags) {
-        uint32_t need_bins = c->status_host[STATUS_BINS_NEEDED];
-        uint32_t need_polys = c->status_host[STATUS_POLYS];
-        // grow for the next call; the frame that overflowed must be re-rendered
-        if (flags & FLAG_BIN_OVERFLOW) {
-            uint64_t want = (uint64_t)need_bins + need_bins / 4 + 1024;
-            int r = ensure(c, c->arena.bins, sizeof(uint32_t) * want);
-            if (r) return r;
-            c->arena.bin_cap = (uint32_t)std::min<uint64_t>(want, 0xFFFFFFF0ull);
-        }
-        if (flags & FLAG_TRIS_OVERFLOW) {
-            uint32_t need = c->status_host[STATUS_TRIS];
-            uint64_t want = std::min<uint64_t>((uint64_t)c->arena.tris_cap + need + need / 4 + 1024, 0xFFFFFFF0ull);
-            int r = ensure(c, c->arena.tris, sizeof(uint4) * want);
-            if (r) return r;
-            c->arena.tris_cap = (uint32_t)want;
-        }
-        if (flags & FLAG_POLY_OVERFLOW) {
-            uint64_t want = (uint64_t)need_polys + need_polys / 4 + 1024;
-            int r;
-            if ((r = ensure(c, c->arena.pool_vr, sizeof(float4) * POLY_MAX * want))) return r;
-            if ((r = ensure(c, c->arena.pool_vs, sizeof(float) * POLY_MAX * POLY_VARY * want))) return r;
-            if ((r = ensure(c, c->arena.pool_face, sizeof(int32_t) * want))) return r;
-            if ((r = ensure(c, c->arena.pool_pass, sizeof(int32_t) * want))) return r;
-            if ((r = ensure(c, c->arena.pool_n, sizeof(int32_t) * want))) return r;
-            c->arena.poly_cap = (uint32_t)want;
-        }
-        return fail(c, HANA_E_OVERFLOW,
-                    "bin or clip-pool capacity exceeded by the last launch; buffers were grown, re-render the frame");
+    // grow for the next call; the frames since the last check must be re-rendered
+    if (flags & FLAG_BIN_OVERFLOW) {
+        uint64_t want = std::max<uint64_t>((uint64_t)need_bins + need_bins / 4 + 1024, c->arena.bin_cap);
+        want = std::min<uint64_t>(want, 0xFFFFFFF0ull);
+        int r = ensure(c, c->arena.bins, sizeof(uint32_t) * want);
+        if (r) return r;
+        c->arena.bin_cap = (uint32_t)want;
     }
-    return HANA_OK;
+    if (flags & FLAG_TRIS_OVERFLOW) {
+        uint64_t want = std::min<uint64_t>((uint64_t)c->arena.tris_cap + need_tris + need_tris / 4 + 1024, 0xFFFFFFF0ull);
+        int r = ensure(c, c->arena.tris, sizeof(uint4) * want);
+        if (r) return r;
+        c->arena.tris_cap = (uint32_t)want;
+    }
+    if (flags & FLAG_POLY_OVERFLOW) {
+        uint64_t want = std::max<uint64_t>((uint64_t)need_polys + need_polys / 4 + 1024, c->arena.poly_cap);
+        want = std::min<uint64_t>(want, (1u << 28) - 1);
+        int r;
+        if ((r = ensure(c, c->arena.pool_vr, sizeof(float4) * POLY_MAX * want))) return r;
+        if ((r = ensure(c, c->arena.pool_vs, sizeof(float) * POLY_MAX * POLY_VARY * want))) return r;
+        if ((r = ensure(c, c->arena.pool_face, sizeof(int32_t) * want))) return r;
+        if ((r = ensure(c, c->arena.pool_pass, sizeof(int32_t) * want))) return r;
+        if ((r = ensure(c, c->arena.pool_n, sizeof(int32_t) * want))) return r;
+        c->arena.poly_cap = (uint32_t)want;
+    }
+    return fail(c, HANA_E_OVERFLOW,
+                "bin or clip-pool capacity exceeded since the last sync; buffers were grown, re-render the frame");
 }
 
 // Renders a list of passes with the context's arena (per-call path).
 int run_passes(hana_ctx* c, std::vector<PassSpec> specs) {
-    int r = check_status(c);
-    if (r) return r;
+    int r;
     for (auto& s : specs) {
         if (!s.shadow || !s.shadow->pending_clear) continue;
         bool drawn = false;
@@ -827,6 +835,10 @@ int hana_batch_finalize(hana_ctx* c, hana_batch* b, int use_graph) {
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         batch_drop_pending(b);
         if (!st[STATUS_FLAGS]) break;
+        HIP_TRY(c, launch_fill32(b->arena.status.as<uint32_t>() + STATUS_SEQ_WORDS, 0u,
+                                 STATUS_WORDS - STATUS_SEQ_WORDS, c->stream));
+        if (st[STATUS_FLAGS] & FLAG_POLY_TOO_LARGE)
+            return fail(c, HANA_E_OVERFLOW, "hana_batch_finalize: a clipped polygon exceeded 10 vertices");
         min_bins = st[STATUS_BINS_NEEDED] + st[STATUS_BINS_NEEDED] / 8 + 1024;
         min_polys = st[STATUS_POLYS] + st[STATUS_POLYS] / 8 + 1024;
         min_tris = st[STATUS_TRIS] + st[STATUS_TRIS] / 8 + 1024;

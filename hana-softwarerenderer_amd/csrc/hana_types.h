@@ -129,10 +129,14 @@ constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_PER_THREAD = 16;
 constexpr int SCAN_CHUNK = SCAN_THREADS * SCAN_PER_THREAD;
 
-// status words: polygons allocated, overflow flags, bin entries needed, faces
-// queued for clipping, clipped fan triangles listed for binning
-enum { STATUS_POLYS = 0, STATUS_FLAGS = 1, STATUS_BINS_NEEDED = 2, STATUS_CLIPQ = 3, STATUS_TRIS = 4,
-       STATUS_WORDS = 5 };
+// status words.  The first STATUS_SEQ_WORDS are counters of one launch
+// sequence (zeroed by its first kernel): polygons allocated, faces queued for
+// clipping, clipped fan triangles listed, bin entries needed.  The others
+// stay set across sequences until the host reads them at a sync point: the
+// overflow flags and the largest needs seen, so that calls need not wait for
+// one another.
+enum { STATUS_POLYS = 0, STATUS_CLIPQ = 1, STATUS_TRIS = 2, STATUS_BINS_NEEDED = 3, STATUS_SEQ_WORDS = 4,
+       STATUS_FLAGS = 4, STATUS_NEED_BINS = 5, STATUS_NEED_POLYS = 6, STATUS_NEED_TRIS = 7, STATUS_WORDS = 8 };
 enum { FLAG_BIN_OVERFLOW = 1, FLAG_POLY_OVERFLOW = 2, FLAG_POLY_TOO_LARGE = 4, FLAG_CLIPQ_OVERFLOW = 8,
        FLAG_TRIS_OVERFLOW = 16 };
 // vertex outcode bits

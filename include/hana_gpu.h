@@ -22,7 +22,10 @@
  *
  * A context owns one HIP stream and every device object created on it; calls
  * on one context must be externally synchronised.  Work is enqueued
- * asynchronously; hana_target_read and hana_gpu_sync wait for it.
+ * asynchronously; hana_target_read(_winner), hana_target_present and
+ * hana_gpu_sync wait for it.  If a draw since the previous such wait outgrew
+ * the workspace, the wait returns HANA_E_OVERFLOW after growing it: render
+ * those frames again.
  */
 #ifndef HANA_GPU_H
 #define HANA_GPU_H
