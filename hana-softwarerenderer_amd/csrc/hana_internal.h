@@ -53,7 +53,9 @@ struct PassSpec {
 
 // Grow-only device workspace for launch sequences.
 struct Arena {
-    DevBuf descs, unis, vr, vs, code, pof, tile_count, tile_offset, tile_cursor, bins, tris, seg_count, scan_part;
+    // tables: the pass descriptors, then (at Plan::uni_off) their uniforms,
+    // uploaded by one copy per call
+    DevBuf tables, vr, vs, code, pof, tile_count, tile_offset, tile_cursor, bins, tris, seg_count, scan_part;
     DevBuf pool_vr, pool_vs, pool_face, pool_pass, pool_n, clip_queue, status, stats;
     uint32_t bin_cap = 0, poly_cap = 0, clipq_cap = 0, tris_cap = 0;
 };
@@ -68,6 +70,7 @@ struct Plan {
     int ngroups = 0;
     int group_first[2] = {0, 0}, group_count[2] = {0, 0}, group_blocks[2] = {0, 0};
     size_t total_v = 0, total_vs = 0, total_f = 0;
+    size_t uni_off = 0;   // byte offset of the uniforms in the tables buffer
     int total_tiles = 0;
     Workspace ws{};
 };
@@ -95,6 +98,17 @@ struct hana_ctx {
     uint64_t* stats_host = nullptr;    // pinned, per pass
     int stats_passes = 0;
     bool status_pending = false;
+    // pipelined draws: consecutive draws alternate between two slots, each a
+    // stream with its own workspace ((stream, arena), (stream2, arena2)), so
+    // the geometry stages of one draw overlap the raster of the one before;
+    // each raster waits for the whole previous draw
+    hipStream_t stream2 = nullptr;
+    Arena arena2;
+    uint32_t* status_host2 = nullptr;
+    hipEvent_t slot_done[2] = {};
+    bool slot_used[2] = {};
+    int next_slot = 0;
+    bool pipelined = true;
     // device staging for hana_target_present
     hana::rt::DevBuf present;
     // optional per-stage timing
@@ -119,7 +133,11 @@ namespace rt {
 int fail(hana_ctx* c, int code, const std::string& msg);
 int hip_fail(hana_ctx* c, hipError_t e, const char* what);
 int ensure(hana_ctx* c, DevBuf& b, size_t bytes);
-int materialize_clear(hana_ctx* c, hana_target* t);
+int materialize_clear(hana_ctx* c, hana_target* t, hipStream_t s);
+// Orders the context stream after every draw in flight on the second stream.
+int join(hana_ctx* c);
+// Waits for both streams of the context.
+int sync_all(hana_ctx* c);
 int check_status(hana_ctx* c);
 // Enqueues the launch sequence of a set of passes on the context's stream.
 int run_passes(hana_ctx* c, std::vector<PassSpec> specs);

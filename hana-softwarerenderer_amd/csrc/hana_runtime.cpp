@@ -41,10 +41,22 @@ int hip_fail(hana_ctx* c, hipError_t e, const char* what) {
         if (e_ != hipSuccess) return hip_fail(c, e_, #expr); \
     } while (0)
 
+int sync_all(hana_ctx* c) {
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    if (c->stream2) HIP_TRY(c, hipStreamSynchronize(c->stream2));
+    return HANA_OK;
+}
+
+int join(hana_ctx* c) {
+    if (c->slot_used[1]) HIP_TRY(c, hipStreamWaitEvent(c->stream, c->slot_done[1], 0));
+    return HANA_OK;
+}
+
 int ensure(hana_ctx* c, DevBuf& b, size_t bytes) {
     if (bytes <= b.cap) return HANA_OK;
     if (b.p) {
-        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        int r = sync_all(c);
+        if (r) return r;
         HIP_TRY(c, hipFree(b.p));
         b.p = nullptr;
         b.cap = 0;
@@ -63,13 +75,13 @@ uint32_t pack_bytes(const uint8_t rgba[4]) {
     return (uint32_t)rgba[0] | ((uint32_t)rgba[1] << 8) | ((uint32_t)rgba[2] << 16) | ((uint32_t)rgba[3] << 24);
 }
 
-int materialize_clear(hana_ctx* c, hana_target* t) {
+int materialize_clear(hana_ctx* c, hana_target* t, hipStream_t s) {
     if (!t->pending_clear) return HANA_OK;
     size_t n = (size_t)t->w * t->h;
-    HIP_TRY(c, launch_fill32(t->color, t->clear_rgba, n, c->stream));
+    HIP_TRY(c, launch_fill32(t->color, t->clear_rgba, n, s));
     uint32_t db;
     std::memcpy(&db, &t->clear_depth, 4);
-    HIP_TRY(c, launch_fill32(reinterpret_cast<uint32_t*>(t->depth), db, n, c->stream));
+    HIP_TRY(c, launch_fill32(reinterpret_cast<uint32_t*>(t->depth), db, n, s));
     t->pending_clear = false;
     return HANA_OK;
 }
@@ -152,11 +164,11 @@ int build_plan(hana_ctx* c, std::vector<PassSpec> specs, Plan& pl) {
 
 // Allocates the arena for a plan and resolves the slice pointers.
 int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_polys, uint32_t min_tris,
-              bool with_stats) {
+              bool with_stats, hipStream_t s) {
     int n = (int)pl.descs.size();
     int r;
-    if ((r = ensure(c, A.descs, sizeof(PassDesc) * std::max(n, 1)))) return r;
-    if ((r = ensure(c, A.unis, sizeof(DevUniforms) * std::max(n, 1)))) return r;
+    pl.uni_off = (sizeof(PassDesc) * std::max(n, 1) + 255) & ~(size_t)255;
+    if ((r = ensure(c, A.tables, pl.uni_off + sizeof(DevUniforms) * std::max(n, 1)))) return r;
     if ((r = ensure(c, A.vr, sizeof(float4) * std::max<size_t>(pl.total_v, 1)))) return r;
     if ((r = ensure(c, A.vs, sizeof(float) * std::max<size_t>(pl.total_vs, 1)))) return r;
     if ((r = ensure(c, A.code, std::max<size_t>(pl.total_v, 4)))) return r;
@@ -203,7 +215,7 @@ int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_p
     }
     bool fresh_status = A.status.p == nullptr;
     if ((r = ensure(c, A.status, sizeof(uint32_t) * STATUS_WORDS))) return r;
-    if (fresh_status) HIP_TRY(c, launch_fill32(A.status.as<uint32_t>(), 0u, STATUS_WORDS, c->stream));
+    if (fresh_status) HIP_TRY(c, launch_fill32(A.status.as<uint32_t>(), 0u, STATUS_WORDS, s));
     if ((r = ensure(c, A.stats, sizeof(uint64_t) * ST_WORDS * std::max(n, 1)))) return r;
     for (int i = 0; i < n; i++) {
         PassDesc& d = pl.descs[i];
@@ -212,7 +224,7 @@ int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_p
         d.vs = A.vs.as<float>() + vsoff;
         d.code = A.code.as<uint8_t>() + voff;
         d.poly_of_face = A.pof.as<int32_t>() + fo;
-        d.u = A.unis.as<DevUniforms>() + i;
+        d.u = reinterpret_cast<const DevUniforms*>(A.tables.as<char>() + pl.uni_off) + i;
         d.stats = with_stats ? A.stats.as<uint64_t>() + (size_t)ST_WORDS * i : nullptr;
     }
     Workspace& ws = pl.ws;
@@ -284,8 +296,9 @@ struct StageTimer {
     }
 };
 
-int enqueue_sequence(hana_ctx* c, const Plan& pl, const Arena& A, hipStream_t s, bool timed = true) {
-    const PassDesc* d = A.descs.as<PassDesc>();
+// The geometry stages: vertex, set-up, clipping, tile scan, binning.
+int enqueue_geometry(hana_ctx* c, const Plan& pl, const Arena& A, hipStream_t s, bool timed) {
+    const PassDesc* d = A.tables.as<PassDesc>();
     int n = (int)pl.descs.size();
     {
         StageTimer t(c, 0, timed, s);
@@ -312,6 +325,12 @@ int enqueue_sequence(hana_ctx* c, const Plan& pl, const Arena& A, hipStream_t s,
         HIP_TRY(c, launch_bin(d, pl.ws, pl.fblocks, s));
         t.done(s);
     }
+    return HANA_OK;
+}
+
+// The raster stages, one launch per dependency group.
+int enqueue_raster(hana_ctx* c, const Plan& pl, const Arena& A, hipStream_t s, bool timed) {
+    const PassDesc* d = A.tables.as<PassDesc>();
     for (int g = 0; g < 2; g++)
         if (pl.group_count[g]) {
             StageTimer t(c, 6 + g, timed, s);
@@ -321,17 +340,27 @@ int enqueue_sequence(hana_ctx* c, const Plan& pl, const Arena& A, hipStream_t s,
     return HANA_OK;
 }
 
+int enqueue_sequence(hana_ctx* c, const Plan& pl, const Arena& A, hipStream_t s, bool timed = true) {
+    int r = enqueue_geometry(c, pl, A, s, timed);
+    return r ? r : enqueue_raster(c, pl, A, s, timed);
+}
+
 // Uploads descriptors and uniforms through a pinned staging slot.
-int upload_tables(hana_ctx* c, const Plan& pl, Arena& A) {
+// Stages the plan's descriptors and uniforms in the pinned ring and copies
+// them into the arena's tables buffer (one copy).
+int upload_tables(hana_ctx* c, const Plan& pl, Arena& A, hipStream_t s) {
     size_t nd = sizeof(PassDesc) * pl.descs.size(), nu = sizeof(DevUniforms) * pl.unis.size();
-    size_t need = nd + nu;
+    size_t need = pl.uni_off + nu;
     if (need > c->ring_bytes) {
-        HIP_TRY(c, hipStreamSynchronize(c->stream));
+        int r = sync_all(c);
+        if (r) return r;
         for (int i = 0; i < kRing; i++) {
             if (c->ring_host[i]) HIP_TRY(c, hipHostFree(c->ring_host[i]));
-            HIP_TRY(c, hipHostMalloc(&c->ring_host[i], need * 2));
+            c->ring_host[i] = nullptr;
             c->ring_used[i] = false;
         }
+        c->ring_bytes = 0;
+        for (int i = 0; i < kRing; i++) HIP_TRY(c, hipHostMalloc(&c->ring_host[i], need * 2));
         c->ring_bytes = need * 2;
     }
     int slot = c->ring_next;
@@ -339,95 +368,133 @@ int upload_tables(hana_ctx* c, const Plan& pl, Arena& A) {
     if (c->ring_used[slot]) HIP_TRY(c, hipEventSynchronize(c->ring_event[slot]));
     char* h = static_cast<char*>(c->ring_host[slot]);
     std::memcpy(h, pl.descs.data(), nd);
-    std::memcpy(h + nd, pl.unis.data(), nu);
-    HIP_TRY(c, hipMemcpyAsync(A.descs.p, h, nd, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, hipMemcpyAsync(A.unis.p, h + nd, nu, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(c, hipEventRecord(c->ring_event[slot], c->stream));
+    std::memcpy(h + pl.uni_off, pl.unis.data(), nu);
+    HIP_TRY(c, hipMemcpyAsync(A.tables.p, h, need, hipMemcpyHostToDevice, s));
+    HIP_TRY(c, hipEventRecord(c->ring_event[slot], s));
     c->ring_used[slot] = true;
     return HANA_OK;
 }
 
-// Reads the sticky status of every sequence since the last check (waits for
-// the stream).  On an overflow the buffers are grown to the largest need
-// seen, the sticky words are cleared, and HANA_E_OVERFLOW asks the caller to
-// render again.
+// Grows an arena's bins, clip pool and triangle list to the needs reported
+// by an overflow (capacities never shrink).
+static int grow_arena(hana_ctx* c, Arena& A, uint32_t flags, uint32_t need_bins, uint32_t need_polys,
+                      uint32_t need_tris) {
+    if (flags & FLAG_BIN_OVERFLOW) {
+        uint64_t want = std::max<uint64_t>((uint64_t)need_bins + need_bins / 4 + 1024, A.bin_cap);
+        want = std::min<uint64_t>(want, 0xFFFFFFF0ull);
+        int r = ensure(c, A.bins, sizeof(uint32_t) * want);
+        if (r) return r;
+        A.bin_cap = (uint32_t)want;
+    }
+    if (flags & FLAG_TRIS_OVERFLOW) {
+        uint64_t want = std::min<uint64_t>((uint64_t)A.tris_cap + need_tris + need_tris / 4 + 1024, 0xFFFFFFF0ull);
+        int r = ensure(c, A.tris, sizeof(uint4) * want);
+        if (r) return r;
+        A.tris_cap = (uint32_t)want;
+    }
+    if (flags & FLAG_POLY_OVERFLOW) {
+        uint64_t want = std::max<uint64_t>((uint64_t)need_polys + need_polys / 4 + 1024, A.poly_cap);
+        want = std::min<uint64_t>(want, (1u << 28) - 1);
+        int r;
+        if ((r = ensure(c, A.pool_vr, sizeof(float4) * POLY_MAX * want))) return r;
+        if ((r = ensure(c, A.pool_vs, sizeof(float) * POLY_MAX * POLY_VARY * want))) return r;
+        if ((r = ensure(c, A.pool_face, sizeof(int32_t) * want))) return r;
+        if ((r = ensure(c, A.pool_pass, sizeof(int32_t) * want))) return r;
+        if ((r = ensure(c, A.pool_n, sizeof(int32_t) * want))) return r;
+        A.poly_cap = (uint32_t)want;
+    }
+    return HANA_OK;
+}
+
+// Reads the sticky status of every sequence since the last check, over both
+// draw slots (waits for both streams).  On an overflow both workspaces are
+// grown to the largest need seen, the sticky words are cleared, and
+// HANA_E_OVERFLOW asks the caller to render again.
 int check_status(hana_ctx* c) {
     if (!c->status_pending) return HANA_OK;
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    int r = sync_all(c);
+    if (r) return r;
     c->status_pending = false;
-    uint32_t flags = c->status_host[STATUS_FLAGS];
+    uint32_t* host[2] = {c->status_host, c->status_host2};
+    Arena* arena[2] = {&c->arena, &c->arena2};
+    uint32_t flags = 0, need_bins = 0, need_polys = 0, need_tris = 0;
+    for (int k = 0; k < 2; k++) {
+        if (!host[k]) continue;
+        flags |= host[k][STATUS_FLAGS];
+        need_bins = std::max(need_bins, host[k][STATUS_NEED_BINS]);
+        need_polys = std::max(need_polys, host[k][STATUS_NEED_POLYS]);
+        need_tris = std::max(need_tris, host[k][STATUS_NEED_TRIS]);
+    }
     if (!flags) return HANA_OK;
-    uint32_t need_bins = c->status_host[STATUS_NEED_BINS];
-    uint32_t need_polys = c->status_host[STATUS_NEED_POLYS];
-    uint32_t need_tris = c->status_host[STATUS_NEED_TRIS];
-    HIP_TRY(c, launch_fill32(c->arena.status.as<uint32_t>() + STATUS_SEQ_WORDS, 0u, STATUS_WORDS - STATUS_SEQ_WORDS,
-                             c->stream));
-    std::memset(c->status_host + STATUS_SEQ_WORDS, 0, sizeof(uint32_t) * (STATUS_WORDS - STATUS_SEQ_WORDS));
+    for (int k = 0; k < 2; k++) {
+        if (!host[k]) continue;
+        if (arena[k]->status.p)
+            HIP_TRY(c, launch_fill32(arena[k]->status.as<uint32_t>() + STATUS_SEQ_WORDS, 0u,
+                                     STATUS_WORDS - STATUS_SEQ_WORDS, c->stream));
+        std::memset(host[k] + STATUS_SEQ_WORDS, 0, sizeof(uint32_t) * (STATUS_WORDS - STATUS_SEQ_WORDS));
+    }
+    // the next geometry stages may run on either stream
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (flags & FLAG_POLY_TOO_LARGE)
         return fail(c, HANA_E_OVERFLOW, "a clipped polygon exceeded 10 vertices and was dropped");
     // grow for the next call; the frames since the last check must be re-rendered
-    if (flags & FLAG_BIN_OVERFLOW) {
-        uint64_t want = std::max<uint64_t>((uint64_t)need_bins + need_bins / 4 + 1024, c->arena.bin_cap);
-        want = std::min<uint64_t>(want, 0xFFFFFFF0ull);
-        int r = ensure(c, c->arena.bins, sizeof(uint32_t) * want);
-        if (r) return r;
-        c->arena.bin_cap = (uint32_t)want;
-    }
-    if (flags & FLAG_TRIS_OVERFLOW) {
-        uint64_t want = std::min<uint64_t>((uint64_t)c->arena.tris_cap + need_tris + need_tris / 4 + 1024, 0xFFFFFFF0ull);
-        int r = ensure(c, c->arena.tris, sizeof(uint4) * want);
-        if (r) return r;
-        c->arena.tris_cap = (uint32_t)want;
-    }
-    if (flags & FLAG_POLY_OVERFLOW) {
-        uint64_t want = std::max<uint64_t>((uint64_t)need_polys + need_polys / 4 + 1024, c->arena.poly_cap);
-        want = std::min<uint64_t>(want, (1u << 28) - 1);
-        int r;
-        if ((r = ensure(c, c->arena.pool_vr, sizeof(float4) * POLY_MAX * want))) return r;
-        if ((r = ensure(c, c->arena.pool_vs, sizeof(float) * POLY_MAX * POLY_VARY * want))) return r;
-        if ((r = ensure(c, c->arena.pool_face, sizeof(int32_t) * want))) return r;
-        if ((r = ensure(c, c->arena.pool_pass, sizeof(int32_t) * want))) return r;
-        if ((r = ensure(c, c->arena.pool_n, sizeof(int32_t) * want))) return r;
-        c->arena.poly_cap = (uint32_t)want;
-    }
+    for (int k = 0; k < 2; k++)
+        if ((r = grow_arena(c, *arena[k], flags, need_bins, need_polys, need_tris))) return r;
     return fail(c, HANA_E_OVERFLOW,
                 "bin or clip-pool capacity exceeded since the last sync; buffers were grown, re-render the frame");
 }
 
-// Renders a list of passes with the context's arena (per-call path).
+// Renders a list of passes (per-call path).  Consecutive calls alternate
+// between two slots, each a stream with its own workspace: the geometry
+// stages of a call run while the previous call rasterises, and its raster
+// waits for the previous call to finish, so targets and shadow maps see the
+// calls in order.  With timing or statistics on, every call uses slot 0.
 int run_passes(hana_ctx* c, std::vector<PassSpec> specs) {
     int r;
+    int a = (c->pipelined && c->stream2 && !c->timing && !c->stats_enabled) ? c->next_slot : 0;
+    hipStream_t S = a ? c->stream2 : c->stream;
+    Arena& A = a ? c->arena2 : c->arena;
+    uint32_t* status_host = a ? c->status_host2 : c->status_host;
+    std::vector<hana_target*> sampled;   // sampled as a shadow map, not drawn here
     for (auto& s : specs) {
         if (!s.shadow || !s.shadow->pending_clear) continue;
         bool drawn = false;
         for (auto& o : specs) drawn = drawn || o.target == s.shadow;
-        if (!drawn && (r = materialize_clear(c, s.shadow))) return r;   // sampled, not drawn here
+        if (!drawn) sampled.push_back(s.shadow);
     }
     Plan pl;
     if ((r = build_plan(c, std::move(specs), pl))) return r;
-    if ((r = bind_plan(c, pl, c->arena, 0, 0, 0, c->stats_enabled))) return r;
+    if ((r = bind_plan(c, pl, A, 0, 0, 0, c->stats_enabled, S))) return r;
     // a target sampled as a shadow map by a pass of the same plan must be drawn
     // by an earlier group; clears fuse into the first pass writing the target
     take_clears(pl);
-    if ((r = upload_tables(c, pl, c->arena))) return r;
+    if ((r = upload_tables(c, pl, A, S))) return r;
     int n = (int)pl.descs.size();
     if (c->stats_enabled) {
-        HIP_TRY(c, launch_fill32(reinterpret_cast<uint32_t*>(c->arena.stats.p), 0u, (size_t)ST_WORDS * 2 * n,
-                                 c->stream));
+        HIP_TRY(c, launch_fill32(reinterpret_cast<uint32_t*>(A.stats.p), 0u, (size_t)ST_WORDS * 2 * n, S));
     }
-    if ((r = enqueue_sequence(c, pl, c->arena, c->stream))) return r;
-    HIP_TRY(c, hipMemcpyAsync(c->status_host, c->arena.status.p, sizeof(uint32_t) * STATUS_WORDS,
-                              hipMemcpyDeviceToHost, c->stream));
+    if ((r = enqueue_geometry(c, pl, A, S, true))) return r;
+    // every status word is final after binning; copied before the wait so the
+    // copy stays off the path from one raster to the next
+    HIP_TRY(c, hipMemcpyAsync(status_host, A.status.p, sizeof(uint32_t) * STATUS_WORDS, hipMemcpyDeviceToHost, S));
+    if (c->slot_used[1 - a]) HIP_TRY(c, hipStreamWaitEvent(S, c->slot_done[1 - a], 0));
+    for (hana_target* t : sampled)
+        if ((r = materialize_clear(c, t, S))) return r;
+    if ((r = enqueue_raster(c, pl, A, S, true))) return r;
+    HIP_TRY(c, hipEventRecord(c->slot_done[a], S));
     if (c->stats_enabled) {
         if (c->stats_passes < n) {
             if (c->stats_host) HIP_TRY(c, hipHostFree(c->stats_host));
+            c->stats_host = nullptr;
+            c->stats_passes = 0;
             HIP_TRY(c, hipHostMalloc((void**)&c->stats_host, sizeof(uint64_t) * ST_WORDS * n));
-            c->stats_passes = n;
         }
-        HIP_TRY(c, hipMemcpyAsync(c->stats_host, c->arena.stats.p, sizeof(uint64_t) * ST_WORDS * n,
-                                  hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(c->stats_host, A.stats.p, sizeof(uint64_t) * ST_WORDS * n,
+                                  hipMemcpyDeviceToHost, S));
         c->stats_passes = n;
     }
+    c->slot_used[a] = true;
+    c->next_slot = 1 - a;
     c->status_pending = true;
     return HANA_OK;
 }
@@ -454,7 +521,12 @@ int hana_gpu_create(int device, hana_ctx** out) {
     auto undo = [c]() {
         for (int i = 0; i < kRing; i++)
             if (c->ring_event[i]) (void)hipEventDestroy(c->ring_event[i]);
+        for (int i = 0; i < 2; i++)
+            if (c->slot_done[i]) (void)hipEventDestroy(c->slot_done[i]);
+        if (c->status_host) (void)hipHostFree(c->status_host);
+        if (c->status_host2) (void)hipHostFree(c->status_host2);
         if (c->stream) (void)hipStreamDestroy(c->stream);
+        if (c->stream2) (void)hipStreamDestroy(c->stream2);
         (void)hipGetLastError();
         delete c;
         return HANA_E_DEVICE;
@@ -465,12 +537,17 @@ int hana_gpu_create(int device, hana_ctx** out) {
         if (hipEventCreateWithFlags(&c->ring_event[i], hipEventDisableTiming) != hipSuccess) return undo();
     if (hipHostMalloc((void**)&c->status_host, sizeof(uint32_t) * STATUS_WORDS) != hipSuccess) return undo();
     std::memset(c->status_host, 0, sizeof(uint32_t) * STATUS_WORDS);
+    if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) return undo();
+    for (int i = 0; i < 2; i++)
+        if (hipEventCreateWithFlags(&c->slot_done[i], hipEventDisableTiming) != hipSuccess) return undo();
+    if (hipHostMalloc((void**)&c->status_host2, sizeof(uint32_t) * STATUS_WORDS) != hipSuccess) return undo();
+    std::memset(c->status_host2, 0, sizeof(uint32_t) * STATUS_WORDS);
     *out = c;
     return HANA_OK;
 }
 
 static void free_arena(Arena& a) {
-    DevBuf* bufs[] = {&a.descs, &a.unis, &a.vr, &a.vs, &a.code, &a.tris, &a.seg_count, &a.scan_part, &a.pof, &a.tile_count, &a.tile_offset,
+    DevBuf* bufs[] = {&a.tables, &a.vr, &a.vs, &a.code, &a.tris, &a.seg_count, &a.scan_part, &a.pof, &a.tile_count, &a.tile_offset,
                       &a.tile_cursor, &a.bins, &a.pool_vr, &a.pool_vs, &a.pool_face, &a.pool_pass, &a.pool_n,
                       &a.clip_queue, &a.status, &a.stats};
     for (DevBuf* b : bufs)
@@ -485,20 +562,26 @@ int hana_gpu_destroy(hana_ctx* c) {
     if (!c) return HANA_E_INVALID;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->stream2);
     free_arena(c->arena);
+    free_arena(c->arena2);
     if (c->present.p) (void)hipFree(c->present.p);
     for (int i = 0; i < kRing; i++) {
         if (c->ring_host[i]) (void)hipHostFree(c->ring_host[i]);
         if (c->ring_event[i]) (void)hipEventDestroy(c->ring_event[i]);
     }
     if (c->status_host) (void)hipHostFree(c->status_host);
+    if (c->status_host2) (void)hipHostFree(c->status_host2);
     if (c->stats_host) (void)hipHostFree(c->stats_host);
+    for (int i = 0; i < 2; i++)
+        if (c->slot_done[i]) (void)hipEventDestroy(c->slot_done[i]);
     for (auto& p : c->ev_pending) {
         (void)hipEventDestroy(p.second.first);
         (void)hipEventDestroy(p.second.second);
     }
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
+    (void)hipStreamDestroy(c->stream2);
     delete c;
     return HANA_OK;
 }
@@ -507,19 +590,20 @@ const char* hana_gpu_last_error(hana_ctx* c) { return c ? c->err.c_str() : "no c
 
 int hana_gpu_sync(hana_ctx* c) {
     if (!c) return HANA_E_INVALID;
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    return check_status(c);
+    int r = sync_all(c);
+    return r ? r : check_status(c);
 }
 
 int hana_gpu_set_timing(hana_ctx* c, int enable) {
     if (!c) return HANA_E_INVALID;
     c->timing = enable != 0;
-    return HANA_OK;
+    return sync_all(c);   // stage times never overlap a draw of the second slot
 }
 
 int hana_gpu_read_timing(hana_ctx* c, double* ms, int64_t* launches) {
     if (!c || !ms || !launches) return HANA_E_INVALID;
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    int r = sync_all(c);
+    if (r) return r;
     for (int i = 0; i < HANA_TIMING_STAGES; i++) {
         ms[i] = 0.0;
         launches[i] = 0;
@@ -544,7 +628,8 @@ int hana_gpu_set_stats(hana_ctx* c, int enable) {
 
 int hana_gpu_last_stats(hana_ctx* c, hana_pass_stats* out, int max_passes, int* n_passes) {
     if (!c || !out || !n_passes) return HANA_E_INVALID;
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    int r = sync_all(c);
+    if (r) return r;
     int n = std::min(max_passes, c->stats_host ? c->stats_passes : 0);
     for (int i = 0; i < n; i++) {
         const uint64_t* s = c->stats_host + (size_t)ST_WORDS * i;
@@ -598,7 +683,7 @@ int hana_mesh_create(hana_ctx* c, int32_t nverts, const float* pos, const float*
 
 int hana_mesh_destroy(hana_ctx* c, hana_mesh* m) {
     if (!c || !m) return HANA_E_INVALID;
-    (void)hipStreamSynchronize(c->stream);
+    (void)sync_all(c);
     (void)hipFree(m->soa);
     (void)hipFree(m->idx);
     delete m;
@@ -635,7 +720,7 @@ int hana_texture_create(hana_ctx* c, int32_t w, int32_t h, int32_t bpp, const ui
 
 int hana_texture_destroy(hana_ctx* c, hana_texture* t) {
     if (!c || !t) return HANA_E_INVALID;
-    (void)hipStreamSynchronize(c->stream);
+    (void)sync_all(c);
     (void)hipFree(t->texels);
     delete t;
     return HANA_OK;
@@ -665,7 +750,7 @@ int hana_target_create(hana_ctx* c, int32_t w, int32_t h, hana_target** out) {
 
 int hana_target_destroy(hana_ctx* c, hana_target* t) {
     if (!c || !t) return HANA_E_INVALID;
-    (void)hipStreamSynchronize(c->stream);
+    (void)sync_all(c);
     (void)hipFree(t->color);
     (void)hipFree(t->depth);
     if (t->winner) (void)hipFree(t->winner);
@@ -684,8 +769,8 @@ int hana_target_clear(hana_ctx* c, hana_target* t, const uint8_t rgba[4], float 
 int hana_target_upload(hana_ctx* c, hana_target* t, const uint8_t* rgba, const float* depth) {
     if (!c || !t) return HANA_E_INVALID;
     size_t n = (size_t)t->w * t->h;
-    int r = materialize_clear(c, t);
-    if (r) return r;
+    int r = join(c);
+    if (r || (r = materialize_clear(c, t, c->stream))) return r;
     if (rgba) HIP_TRY(c, hipMemcpyAsync(t->color, rgba, 4 * n, hipMemcpyHostToDevice, c->stream));
     if (depth) HIP_TRY(c, hipMemcpyAsync(t->depth, depth, 4 * n, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -694,8 +779,8 @@ int hana_target_upload(hana_ctx* c, hana_target* t, const uint8_t* rgba, const f
 
 int hana_target_read(hana_ctx* c, hana_target* t, uint8_t* rgba, float* depth) {
     if (!c || !t) return HANA_E_INVALID;
-    int r = materialize_clear(c, t);
-    if (r) return r;
+    int r = join(c);
+    if (r || (r = materialize_clear(c, t, c->stream))) return r;
     size_t n = (size_t)t->w * t->h;
     if (rgba) HIP_TRY(c, hipMemcpyAsync(rgba, t->color, 4 * n, hipMemcpyDeviceToHost, c->stream));
     if (depth) HIP_TRY(c, hipMemcpyAsync(depth, t->depth, 4 * n, hipMemcpyDeviceToHost, c->stream));
@@ -705,8 +790,8 @@ int hana_target_read(hana_ctx* c, hana_target* t, uint8_t* rgba, float* depth) {
 
 int hana_target_present(hana_ctx* c, hana_target* t, uint8_t* bgra) {
     if (!c || !t || !bgra) return fail(c, HANA_E_INVALID, "hana_target_present: null argument");
-    int r = materialize_clear(c, t);
-    if (r) return r;
+    int r = join(c);
+    if (r || (r = materialize_clear(c, t, c->stream))) return r;
     size_t n = (size_t)t->w * t->h;
     if ((r = ensure(c, c->present, 4 * n))) return r;
     uint32_t* dst = c->present.as<uint32_t>();
@@ -725,8 +810,9 @@ int hana_target_enable_winner(hana_ctx* c, hana_target* t, int enable) {
             return fail(c, HANA_E_NOMEM, "winner map allocation failed");
         }
         HIP_TRY(c, launch_fill32(t->winner, NO_WINNER, n, c->stream));
+        HIP_TRY(c, hipStreamSynchronize(c->stream));   // draws of either slot may use it next
     } else if (!enable && t->winner) {
-        (void)hipStreamSynchronize(c->stream);
+        (void)sync_all(c);
         (void)hipFree(t->winner);
         t->winner = nullptr;
     }
@@ -736,6 +822,8 @@ int hana_target_enable_winner(hana_ctx* c, hana_target* t, int enable) {
 int hana_target_read_winner(hana_ctx* c, hana_target* t, uint32_t* winner) {
     if (!c || !t || !winner || !t->winner) return HANA_E_INVALID;
     size_t n = (size_t)t->w * t->h;
+    int r = join(c);
+    if (r) return r;
     HIP_TRY(c, hipMemcpyAsync(winner, t->winner, 4 * n, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return HANA_OK;
@@ -818,16 +906,16 @@ int hana_batch_finalize(hana_ctx* c, hana_batch* b, int use_graph) {
             if (b->specs[i].target == b->specs[j].target)
                 return fail(c, HANA_E_INVALID, "hana_batch_finalize: frames of a batch need distinct targets and shadow maps");
     int r;
-    if ((r = check_status(c))) return r;
+    if ((r = check_status(c)) || (r = join(c))) return r;
     if ((r = build_plan(c, b->specs, b->plan))) return r;
     uint32_t min_bins = 0, min_polys = 0, min_tris = 0;
     for (int attempt = 0; attempt < 3; attempt++) {
-        if ((r = bind_plan(c, b->plan, b->arena, min_bins, min_polys, min_tris, false))) return r;
+        if ((r = bind_plan(c, b->plan, b->arena, min_bins, min_polys, min_tris, false, c->stream))) return r;
         batch_clear_flags(b);
-        HIP_TRY(c, hipMemcpy(b->arena.descs.p, b->plan.descs.data(), sizeof(PassDesc) * b->plan.descs.size(),
+        HIP_TRY(c, hipMemcpy(b->arena.tables.p, b->plan.descs.data(), sizeof(PassDesc) * b->plan.descs.size(),
                              hipMemcpyHostToDevice));
-        HIP_TRY(c, hipMemcpy(b->arena.unis.p, b->plan.unis.data(), sizeof(DevUniforms) * b->plan.unis.size(),
-                             hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(b->arena.tables.as<char>() + b->plan.uni_off, b->plan.unis.data(),
+                             sizeof(DevUniforms) * b->plan.unis.size(), hipMemcpyHostToDevice));
         // trial run: sizes the bins and the clip pool exactly
         if ((r = enqueue_sequence(c, b->plan, b->arena, c->stream))) return r;
         uint32_t st[STATUS_WORDS];
@@ -862,6 +950,9 @@ int hana_batch_finalize(hana_ctx* c, hana_batch* b, int use_graph) {
 int hana_batch_run(hana_ctx* c, hana_batch* b) {
     if (!c || !b || !b->finalized) return fail(c, HANA_E_INVALID, "hana_batch_run: batch not finalized");
     batch_drop_pending(b);
+    int r = join(c);
+    if (r) return r;
+    c->next_slot = 0;   // the next draw follows the sequence on the same stream
     if (b->use_graph) {
         HIP_TRY(c, hipGraphLaunch(b->exec, c->stream));
         return HANA_OK;
@@ -871,7 +962,7 @@ int hana_batch_run(hana_ctx* c, hana_batch* b) {
 
 int hana_batch_destroy(hana_ctx* c, hana_batch* b) {
     if (!c || !b) return HANA_E_INVALID;
-    (void)hipStreamSynchronize(c->stream);
+    (void)sync_all(c);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     free_arena(b->arena);

@@ -311,10 +311,17 @@ int hana_server_create(hana_ctx* ctx, const hana_server_config* cfg, hana_server
         Lane L;
         if (hana_gpu_create(ctx->device, &L.ctx) != HANA_OK ||
             hipEventCreateWithFlags(&L.done, hipEventDisableTiming) != hipSuccess) {
-            for (Lane& l : s->lanes) hana_gpu_destroy(l.ctx);
+            if (L.ctx) hana_gpu_destroy(L.ctx);
+            for (Lane& l : s->lanes) {
+                hana_gpu_destroy(l.ctx);
+                (void)hipEventDestroy(l.done);
+            }
             delete s;
             return fail(ctx, HANA_E_DEVICE, "hana_server_create: lane set-up failed");
         }
+        // a lane's completion event is recorded on its context stream, so its
+        // draws stay on that one stream
+        L.ctx->pipelined = false;
         s->lanes.push_back(L);
     }
     s->worker = std::thread(worker_main, s);

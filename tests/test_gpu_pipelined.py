@@ -1,0 +1,117 @@
+"""Consecutive draws with no read in between.  The per-call path alternates
+two streams with their own workspaces, so a draw's geometry stages overlap
+the previous draw's raster; the results must still be those of the calls
+run one after another: depth tests across draws, shadow maps written by one
+call and sampled by the next, targets read, uploaded or presented between
+calls, and timing or statistics switched on in the middle of a chain."""
+import numpy as np
+import pytest
+
+from hana_amd import scenegen as sg
+from parity import assert_color_ok, assert_depth_exact
+from pyoracle import CPURenderer, HostTarget
+
+pytestmark = pytest.mark.gpu
+
+W, H = 150, 110
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from hana_amd.gpu import Context
+    c = Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def oracle():
+    return CPURenderer("oracle")
+
+
+def chain_scenes(n, seed0):
+    kinds = [sg.NORMALMAP, sg.BLINN, sg.TEXTURE_LIGHT, sg.TOON, sg.GROUND, sg.TEXTURE]
+    cams = [(0.0, 0.0, 2.0), (0.4, 0.3, 1.8), (-0.5, 0.1, 1.6), (0.2, -0.6, 2.2)]
+    return [sg.small_scene(kinds[i % len(kinds)], seed=seed0 + i, width=W, height=H, cam=cams[i % len(cams)])
+            for i in range(n)]
+
+
+def oracle_chain(oracle, scenes, target=None):
+    t = target or HostTarget(W, H)
+    for sc in scenes:
+        oracle.draw(sc.mesh, sc.shader, oracle.build_uniforms(sc), t, sc.diffuse, sc.normal)
+    return t
+
+
+@pytest.mark.parametrize("n", [2, 3, 7])
+def test_draw_chain_into_one_target(ctx, oracle, n):
+    """n draws of different meshes into one target, depth-tested against each other."""
+    from hana_amd.gpu import build_uniforms
+    scenes = chain_scenes(n, 300 + n)
+    tgt = ctx.target(W, H)
+    keep = []
+    for sc in scenes:
+        u = build_uniforms(sc)
+        keep.append(u)
+        ctx.draw(ctx.mesh(sc.mesh), sc.shader, u, tgt, ctx.texture(sc.diffuse), ctx.texture(sc.normal))
+    col, dep = ctx.read(tgt, W, H)
+    ref = oracle_chain(oracle, scenes)
+    assert_depth_exact(dep, ref.depth, f"chain{n}")
+    assert_color_ok(col, ref.color, f"chain{n}")
+
+
+def test_alternating_frames_with_shadows(ctx, oracle):
+    """Two shadowed scenes rendered in turn, each frame drawing its shadow map
+    and sampling it in the next call; read only at the end."""
+    from hana_amd.gpu import SceneOnDevice
+    scenes = [sg.small_scene(k, shadow=True, seed=310 + k, width=W, height=H) for k in (sg.NORMALMAP, sg.BLINN)]
+    devs = [SceneOnDevice(ctx, sc) for sc in scenes]
+    for _ in range(3):
+        for d in devs:
+            d.clear()
+            d.render()
+    for sc, d in zip(scenes, devs):
+        col, dep = d.read()
+        ref, _, _ = oracle.render_frame(sc)
+        assert_depth_exact(dep, ref.depth, sc.name)
+        assert_color_ok(col, ref.color, sc.name)
+
+
+def test_reads_uploads_and_presents_between_draws(ctx, oracle):
+    """Target operations on the context stream are ordered after draws of
+    either slot, and later draws after them."""
+    from hana_amd.gpu import build_uniforms
+    scenes = chain_scenes(5, 320)
+    tgt = ctx.target(W, H)
+    ref = HostTarget(W, H)
+    keep = []
+
+    def draw(i):
+        sc = scenes[i]
+        u = build_uniforms(sc)
+        keep.append(u)
+        ctx.draw(ctx.mesh(sc.mesh), sc.shader, u, tgt, ctx.texture(sc.diffuse), ctx.texture(sc.normal))
+        oracle_chain(oracle, [sc], ref)
+
+    draw(0)
+    draw(1)
+    col, dep = ctx.read(tgt, W, H)   # after a draw on the second slot
+    assert_depth_exact(dep, ref.depth, "read1")
+    draw(2)
+    out = ctx.present(tgt, W, H)
+    assert_color_ok(out[::-1][..., [2, 1, 0, 3]], ref.color, "present")
+    # replace the target's contents, then keep drawing over them
+    rng = np.random.default_rng(5)
+    up_c = rng.integers(0, 256, (H, W, 4), dtype=np.uint8)
+    up_d = rng.uniform(0.2, 1.0, (H, W)).astype(np.float32)
+    ctx.upload(tgt, up_c, up_d)
+    ref.color[...] = up_c
+    ref.depth[...] = up_d
+    draw(3)
+    ctx.set_timing(True)   # switches to one stream mid-chain
+    draw(4)
+    ctx.set_timing(False)
+    ctx.read_timing()
+    col, dep = ctx.read(tgt, W, H)
+    assert_depth_exact(dep, ref.depth, "after upload")
+    assert_color_ok(col, ref.color, "after upload")
