@@ -20,12 +20,15 @@
  *   hana_tga_load/save   TGAImage::read_tga_file / write_tga_file   SR/tgaimage.cpp:40-246
  *   HANA_SHADER_*        the IShader subclasses                     SR/IShader.h:132-165
  *
- * A context owns one HIP stream and every device object created on it; calls
- * on one context must be externally synchronised.  Work is enqueued
- * asynchronously; hana_target_read(_winner), hana_target_present and
- * hana_gpu_sync wait for it.  If a draw since the previous such wait outgrew
- * the workspace, the wait returns HANA_E_OVERFLOW after growing it: render
- * those frames again.
+ * A context owns two HIP streams, each with its own workspace, and every
+ * device object created on it; calls on one context must be externally
+ * synchronised.  Work is enqueued asynchronously: consecutive draws alternate
+ * between the two streams so that one draw's geometry stages overlap the
+ * previous draw's raster, and every raster waits for the previous draw, so
+ * targets see the draws in call order.  hana_target_read(_winner),
+ * hana_target_present and hana_gpu_sync wait for all of it.  If a draw since
+ * the previous such wait outgrew the workspace, the wait returns
+ * HANA_E_OVERFLOW after growing it: render those frames again.
  */
 #ifndef HANA_GPU_H
 #define HANA_GPU_H
@@ -123,8 +126,8 @@ int hana_gpu_sync(hana_ctx* ctx);
 int hana_gpu_set_stats(hana_ctx* ctx, int enable);
 int hana_gpu_last_stats(hana_ctx* ctx, hana_pass_stats* out, int max_passes, int* n_passes);
 
-/* Per-stage kernel timing with HIP events recorded on the context stream
- * around each launch (benchmarking).  Stages: 0 vertex, 1 set-up (cull,
+/* Per-stage kernel timing with HIP events recorded around each launch
+ * (benchmarking; while it is on, draws use one stream and do not overlap).  Stages: 0 vertex, 1 set-up (cull,
  * bounds, tile counts), 2 clipping, 3 tile scan, 4 binning, 5 (unused, no
  * launches), 6 raster of the first dependency group (shadow passes), 7 raster
  * of the second group (main passes).  Reading returns the milliseconds and
