@@ -433,21 +433,22 @@ __global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* passes, int npa
             ids[k][2] = P.idx[3 * f + 2];
         }
     }
+    // outcodes and post-transform vertices are gathered together (every
+    // vertex has both), so the vertices do not wait for the trivial-accept test
     uint32_t codes[FPT][3];
-#pragma unroll
-    for (int k = 0; k < FPT; k++)
-        for (int j = 0; j < 3; j++) codes[k][j] = valid[k] ? P.code[ids[k][j]] : 0u;
-    bool trivial[FPT];
     float4 va[FPT], vb[FPT], vc[FPT];
 #pragma unroll
     for (int k = 0; k < FPT; k++) {
-        trivial[k] = valid[k] && (codes[k][0] & codes[k][1] & codes[k][2] & CODE_VISIBLE);
-        if (trivial[k]) {
+        for (int j = 0; j < 3; j++) codes[k][j] = valid[k] ? P.code[ids[k][j]] : 0u;
+        if (valid[k]) {
             va[k] = P.vr[ids[k][0]];
             vb[k] = P.vr[ids[k][1]];
             vc[k] = P.vr[ids[k][2]];
         }
     }
+    bool trivial[FPT];
+#pragma unroll
+    for (int k = 0; k < FPT; k++) trivial[k] = valid[k] && (codes[k][0] & codes[k][1] & codes[k][2] & CODE_VISIBLE);
 #pragma unroll
     for (int k = 0; k < FPT; k++) {
         int f = base + k * BLOCK + threadIdx.x;
@@ -759,13 +760,31 @@ __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* sh) {
     return t;
 }
 
+// A thread's SCAN_PER_THREAD consecutive counters from b (a multiple of 16,
+// so whole runs load as 16-byte vectors; buffers are 256-byte aligned).
+__device__ __forceinline__ void load_counts(const uint32_t* c, int b, int n, uint32_t (&v)[SCAN_PER_THREAD]) {
+    if (b + SCAN_PER_THREAD <= n) {
+        const uint4* q = reinterpret_cast<const uint4*>(c + b);
+#pragma unroll
+        for (int i = 0; i < SCAN_PER_THREAD / 4; i++) {
+            uint4 x = q[i];
+            v[4 * i] = x.x; v[4 * i + 1] = x.y; v[4 * i + 2] = x.z; v[4 * i + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < SCAN_PER_THREAD; i++) v[i] = (b + i < n) ? c[b + i] : 0u;
+    }
+}
+
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(Workspace ws) {
     __shared__ uint32_t sh[SCAN_THREADS / 64];
     int n = ws.ntiles;
     int b = blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_PER_THREAD;
+    uint32_t v[SCAN_PER_THREAD];
+    load_counts(ws.tile_count, b, n, v);
     uint32_t s = 0;
-    for (int i = 0; i < SCAN_PER_THREAD; i++)
-        if (b + i < n) s += ws.tile_count[b + i];
+#pragma unroll
+    for (int i = 0; i < SCAN_PER_THREAD; i++) s += v[i];
     uint32_t t = block_sum(s, sh);
     if (threadIdx.x == 0) ws.scan_part[blockIdx.x] = t;
     if (blockIdx.x == 0 && threadIdx.x == 0) {   // set-up and clipping are complete
@@ -786,11 +805,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(Workspace ws) {
     // this thread's counters, then an exclusive scan of the thread sums
     int b = blockIdx.x * SCAN_CHUNK + tid * SCAN_PER_THREAD;
     uint32_t v[SCAN_PER_THREAD];
+    load_counts(ws.tile_count, b, n, v);
     uint32_t s = 0;
-    for (int i = 0; i < SCAN_PER_THREAD; i++) {
-        v[i] = (b + i < n) ? ws.tile_count[b + i] : 0u;
-        s += v[i];
-    }
+#pragma unroll
+    for (int i = 0; i < SCAN_PER_THREAD; i++) s += v[i];
     uint32_t incl = s;
     for (int d = 1; d < 64; d <<= 1) {
         uint32_t o = __shfl_up(incl, d);
@@ -801,12 +819,27 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(Workspace ws) {
     uint32_t wbefore = 0;
     for (int i = 0; i < w; i++) wbefore += wsum[i];
     uint32_t run = before + wbefore + incl - s;
-    for (int i = 0; i < SCAN_PER_THREAD; i++)
-        if (b + i < n) {
-            ws.tile_offset[b + i] = run;
-            ws.tile_cursor[b + i] = run;
-            run += v[i];
+    if (b + SCAN_PER_THREAD <= n) {
+        uint4* qo = reinterpret_cast<uint4*>(ws.tile_offset + b);
+        uint4* qc = reinterpret_cast<uint4*>(ws.tile_cursor + b);
+#pragma unroll
+        for (int i = 0; i < SCAN_PER_THREAD / 4; i++) {
+            uint4 x;
+            x.x = run; run += v[4 * i];
+            x.y = run; run += v[4 * i + 1];
+            x.z = run; run += v[4 * i + 2];
+            x.w = run; run += v[4 * i + 3];
+            qo[i] = x;
+            qc[i] = x;
         }
+    } else {
+        for (int i = 0; i < SCAN_PER_THREAD; i++)
+            if (b + i < n) {
+                ws.tile_offset[b + i] = run;
+                ws.tile_cursor[b + i] = run;
+                run += v[i];
+            }
+    }
     if (blockIdx.x == gridDim.x - 1 && tid == SCAN_THREADS - 1) {
         // `run` of the last thread is the grand total
         ws.tile_offset[n] = run;

@@ -146,6 +146,14 @@ def test_render_frame_semantics(ctx, oracle):
     assert_color_ok(col1, tgt.color, "frame")
 
 
+@pytest.mark.parametrize("wh", [(4096, 4096), (4112, 4096)])
+def test_tile_scan_large_targets(ctx, oracle, wh):
+    """Large targets: 65,536 tiles (16 scan chunks) and 65,792 (a partial
+    17th chunk) give the oracle's frame."""
+    scene = sg.small_scene(sg.BLINN, seed=77, width=wh[0], height=wh[1], segments=48, rings=24)
+    check_scene(ctx, oracle, scene)
+
+
 @pytest.mark.slow
 def test_config2_full(ctx, oracle):
     scene = sg.config2_scene()
