@@ -914,6 +914,7 @@ __device__ __forceinline__ void frag_resolve(TileLds& L, const TriS& t, uint32_t
 __device__ void sweep(TileLds& L, const PassDesc& P, const Workspace& ws, uint32_t beg, uint32_t cnt, int xb, int yb,
                       bool second) {
     int tid = threadIdx.x, lane = tid & 63;
+    bool er = cnt > (uint32_t)HANA_ER_MIN_ENTRIES;   // uniform over the workgroup
     for (uint32_t base = 0; base < cnt; base += RBLOCK) {
         bool ok = false;
         TriS t;
@@ -930,25 +931,23 @@ __device__ void sweep(TileLds& L, const PassDesc& P, const Workspace& ws, uint32
             int wdt = bx1 - bx0 + 1;
             int npix = wdt * (by1 - by0 + 1);
             if (npix <= SMALL_PIX) {
-#if HANA_EARLY_REJECT
-              if (npix <= HANA_ER_MAX_PIX) {
-                // cheap pass over the candidates, then the exact test on the survivors
-                uint32_t mask = 0;
-                int k = 0;
-                for (int ly = by0; ly <= by1; ly++)
-                    for (int lx = bx0; lx <= bx1; lx++, k++)
-                        if (maybe_covered(t, xb + lx, yb + ly)) mask |= 1u << k;
-                uint32_t magic = tile_div_magic(wdt);
-                while (mask) {
-                    int b = __builtin_ctz(mask);
-                    mask &= mask - 1u;
-                    int row = tile_div(b, magic);
-                    frag_resolve(L, t, lo, xb, yb, bx0 + b - row * wdt, by0 + row, second);
-                }
-              } else
-#endif
-                for (int ly = by0; ly <= by1; ly++)
-                    for (int lx = bx0; lx <= bx1; lx++) frag_resolve(L, t, lo, xb, yb, lx, ly, second);
+                if (er && npix <= HANA_ER_MAX_PIX) {
+                    // cheap pass over the candidates, then the exact test on the survivors
+                    uint32_t mask = 0;
+                    int k = 0;
+                    for (int ly = by0; ly <= by1; ly++)
+                        for (int lx = bx0; lx <= bx1; lx++, k++)
+                            if (maybe_covered(t, xb + lx, yb + ly)) mask |= 1u << k;
+                    uint32_t magic = tile_div_magic(wdt);
+                    while (mask) {
+                        int b = __builtin_ctz(mask);
+                        mask &= mask - 1u;
+                        int row = tile_div(b, magic);
+                        frag_resolve(L, t, lo, xb, yb, bx0 + b - row * wdt, by0 + row, second);
+                    }
+                } else
+                    for (int ly = by0; ly <= by1; ly++)
+                        for (int lx = bx0; lx <= bx1; lx++) frag_resolve(L, t, lo, xb, yb, lx, ly, second);
             } else {
                 L.ax[tid] = t.ax; L.ay[tid] = t.ay; L.s0x[tid] = t.s0x; L.s0y[tid] = t.s0y;
                 L.s1x[tid] = t.s1x; L.s1y[tid] = t.s1y; L.uz[tid] = t.uz;

@@ -25,11 +25,15 @@ constexpr int BLOCK = 256;                 // threads per workgroup (4 wave64)
 #ifndef HANA_SMALL_PIX
 #define HANA_SMALL_PIX 32
 #endif
-#ifndef HANA_EARLY_REJECT
-#define HANA_EARLY_REJECT 0
-#endif
+// The division-free reject pass runs for small triangles of up to
+// HANA_ER_MAX_PIX candidates, in tiles whose bin holds more than
+// HANA_ER_MIN_ENTRIES triangles (dense micro-triangle tiles, where many
+// candidates fail; in sparse tiles the extra pass costs more than it saves).
 #ifndef HANA_ER_MAX_PIX
-#define HANA_ER_MAX_PIX 32   // the division-free reject pass runs for small triangles up to this many candidates
+#define HANA_ER_MAX_PIX 32
+#endif
+#ifndef HANA_ER_MIN_ENTRIES
+#define HANA_ER_MIN_ENTRIES 320
 #endif
 constexpr int RBLOCK = HANA_RASTER_BLOCK;  // threads per raster workgroup (one tile)
 static_assert(RBLOCK % 64 == 0 && RBLOCK <= TILE_PIX, "raster workgroup: whole waves, at most one lane per pixel");
