@@ -117,14 +117,24 @@ struct hana_ctx {
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
 };
 
-struct hana_batch {
+// A batch's frames are split into up to two parts, each a launch sequence
+// with its own workspace; the parts run concurrently on the context's two
+// streams (two branches of one graph), since their frames share no target.
+struct hana_batch_part {
     std::vector<hana::rt::PassSpec> specs;
     hana::rt::Plan plan;
     hana::rt::Arena arena;
+};
+
+struct hana_batch {
+    std::vector<hana::rt::PassSpec> specs;
+    std::vector<size_t> frame_first;   // index in specs of each frame's first pass
+    std::vector<hana_batch_part> parts;
     bool finalized = false;
     bool use_graph = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    hipEvent_t fork = nullptr, joined = nullptr;
 };
 
 namespace hana {

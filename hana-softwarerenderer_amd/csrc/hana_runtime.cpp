@@ -874,6 +874,7 @@ int hana_batch_add_frame(hana_ctx* c, hana_batch* b, const hana_frame_desc* f) {
     if (b->finalized) return fail(c, HANA_E_INVALID, "hana_batch_add_frame: batch already finalized");
     hana_uniforms u = *f->uniforms;
     u.enable_shadow = f->enable_shadow;
+    b->frame_first.push_back(b->specs.size());
     if (f->enable_shadow) b->specs.push_back({f->mesh, HANA_SHADER_SHADOW, u, f->diffuse, f->normal, nullptr, f->shadow_map, 0});
     b->specs.push_back({f->mesh, f->shader, u, f->diffuse, f->normal, f->enable_shadow ? f->shadow_map : nullptr,
                         f->target, 1});
@@ -882,9 +883,9 @@ int hana_batch_add_frame(hana_ctx* c, hana_batch* b, const hana_frame_desc* f) {
 
 // Every frame of a batch clears its target and shadow map first (the state
 // DrawModel::draw runs in after main.cpp's per-frame clear).
-static void batch_clear_flags(hana_batch* b) {
-    for (size_t i = 0; i < b->plan.descs.size(); i++) {
-        PassDesc& d = b->plan.descs[i];
+static void batch_clear_flags(hana_batch_part& part) {
+    for (size_t i = 0; i < part.plan.descs.size(); i++) {
+        PassDesc& d = part.plan.descs[i];
         d.clear = 1;
         d.clear_rgba = 0xFF000000u;
         d.clear_depth = 3.40282347e+38f;
@@ -898,47 +899,112 @@ static void batch_drop_pending(hana_batch* b) {
     for (auto& s : b->specs) s.target->pending_clear = false;
 }
 
-int hana_batch_finalize(hana_ctx* c, hana_batch* b, int use_graph) {
-    if (!c || !b) return HANA_E_INVALID;
-    if (b->specs.empty()) return fail(c, HANA_E_INVALID, "hana_batch_finalize: empty batch");
-    for (size_t i = 0; i < b->specs.size(); i++)
-        for (size_t j = i + 1; j < b->specs.size(); j++)
-            if (b->specs[i].target == b->specs[j].target)
-                return fail(c, HANA_E_INVALID, "hana_batch_finalize: frames of a batch need distinct targets and shadow maps");
+// Splits the frames into two parts of about equal work (faces plus an eighth
+// of the pixels drawn), or keeps one part for a single frame.
+static void batch_split(hana_batch* b) {
+    size_t nf = b->frame_first.size();
+    for (auto& part : b->parts) free_arena(part.arena);   // from a finalize that failed
+    b->parts.clear();
+    b->parts.resize(nf >= 2 ? 2 : 1);
+    std::vector<double> w(nf, 0.0);
+    double total = 0.0;
+    for (size_t k = 0; k < nf; k++) {
+        size_t e = k + 1 < nf ? b->frame_first[k + 1] : b->specs.size();
+        for (size_t i = b->frame_first[k]; i < e; i++)
+            w[k] += (double)b->specs[i].mesh->F + (double)b->specs[i].target->w * b->specs[i].target->h / 8.0;
+        total += w[k];
+    }
+    size_t split = nf;
+    if (b->parts.size() == 2) {
+        double run = 0.0;
+        split = 1;
+        for (size_t k = 0; k + 1 < nf; k++) {
+            run += w[k];
+            split = k + 1;
+            if (run >= total / 2) break;
+        }
+    }
+    size_t cut = split < nf ? b->frame_first[split] : b->specs.size();
+    b->parts[0].specs.assign(b->specs.begin(), b->specs.begin() + cut);
+    if (b->parts.size() == 2) b->parts[1].specs.assign(b->specs.begin() + cut, b->specs.end());
+}
+
+// Enqueues every part: with two parts, part 1 runs on the second stream,
+// forked from and joined back into the context stream (with stage timing
+// on, both run on the context stream so that stage times do not overlap).
+static int batch_enqueue(hana_ctx* c, hana_batch* b, bool timed) {
     int r;
-    if ((r = check_status(c)) || (r = join(c))) return r;
-    if ((r = build_plan(c, b->specs, b->plan))) return r;
+    if (b->parts.size() == 1 || (timed && c->timing)) {
+        for (auto& part : b->parts)
+            if ((r = enqueue_sequence(c, part.plan, part.arena, c->stream, timed))) return r;
+        return HANA_OK;
+    }
+    HIP_TRY(c, hipEventRecord(b->fork, c->stream));
+    HIP_TRY(c, hipStreamWaitEvent(c->stream2, b->fork, 0));
+    if ((r = enqueue_sequence(c, b->parts[0].plan, b->parts[0].arena, c->stream, timed))) return r;
+    if ((r = enqueue_sequence(c, b->parts[1].plan, b->parts[1].arena, c->stream2, timed))) return r;
+    HIP_TRY(c, hipEventRecord(b->joined, c->stream2));
+    HIP_TRY(c, hipStreamWaitEvent(c->stream, b->joined, 0));
+    return HANA_OK;
+}
+
+// Sizes a part's workspace exactly with trial runs.
+static int batch_size_part(hana_ctx* c, hana_batch* b, hana_batch_part& part) {
+    int r;
+    if ((r = build_plan(c, part.specs, part.plan))) return r;
     uint32_t min_bins = 0, min_polys = 0, min_tris = 0;
     for (int attempt = 0; attempt < 3; attempt++) {
-        if ((r = bind_plan(c, b->plan, b->arena, min_bins, min_polys, min_tris, false, c->stream))) return r;
-        batch_clear_flags(b);
-        HIP_TRY(c, hipMemcpy(b->arena.tables.p, b->plan.descs.data(), sizeof(PassDesc) * b->plan.descs.size(),
+        if ((r = bind_plan(c, part.plan, part.arena, min_bins, min_polys, min_tris, false, c->stream))) return r;
+        batch_clear_flags(part);
+        HIP_TRY(c, hipMemcpy(part.arena.tables.p, part.plan.descs.data(), sizeof(PassDesc) * part.plan.descs.size(),
                              hipMemcpyHostToDevice));
-        HIP_TRY(c, hipMemcpy(b->arena.tables.as<char>() + b->plan.uni_off, b->plan.unis.data(),
-                             sizeof(DevUniforms) * b->plan.unis.size(), hipMemcpyHostToDevice));
+        HIP_TRY(c, hipMemcpy(part.arena.tables.as<char>() + part.plan.uni_off, part.plan.unis.data(),
+                             sizeof(DevUniforms) * part.plan.unis.size(), hipMemcpyHostToDevice));
         // trial run: sizes the bins and the clip pool exactly
-        if ((r = enqueue_sequence(c, b->plan, b->arena, c->stream))) return r;
+        if ((r = enqueue_sequence(c, part.plan, part.arena, c->stream))) return r;
         uint32_t st[STATUS_WORDS];
-        HIP_TRY(c, hipMemcpyAsync(st, b->arena.status.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(st, part.arena.status.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         batch_drop_pending(b);
-        if (!st[STATUS_FLAGS]) break;
-        HIP_TRY(c, launch_fill32(b->arena.status.as<uint32_t>() + STATUS_SEQ_WORDS, 0u,
+        if (!st[STATUS_FLAGS]) return HANA_OK;
+        HIP_TRY(c, launch_fill32(part.arena.status.as<uint32_t>() + STATUS_SEQ_WORDS, 0u,
                                  STATUS_WORDS - STATUS_SEQ_WORDS, c->stream));
         if (st[STATUS_FLAGS] & FLAG_POLY_TOO_LARGE)
             return fail(c, HANA_E_OVERFLOW, "hana_batch_finalize: a clipped polygon exceeded 10 vertices");
         min_bins = st[STATUS_BINS_NEEDED] + st[STATUS_BINS_NEEDED] / 8 + 1024;
         min_polys = st[STATUS_POLYS] + st[STATUS_POLYS] / 8 + 1024;
         min_tris = st[STATUS_TRIS] + st[STATUS_TRIS] / 8 + 1024;
-        if (attempt == 2) return fail(c, HANA_E_OVERFLOW, "hana_batch_finalize: capacity not reached");
+    }
+    return fail(c, HANA_E_OVERFLOW, "hana_batch_finalize: capacity not reached");
+}
+
+int hana_batch_finalize(hana_ctx* c, hana_batch* b, int use_graph) {
+    if (!c || !b) return HANA_E_INVALID;
+    if (b->specs.empty()) return fail(c, HANA_E_INVALID, "hana_batch_finalize: empty batch");
+    if (b->finalized) return fail(c, HANA_E_INVALID, "hana_batch_finalize: batch already finalized");
+    for (size_t i = 0; i < b->specs.size(); i++)
+        for (size_t j = i + 1; j < b->specs.size(); j++)
+            if (b->specs[i].target == b->specs[j].target)
+                return fail(c, HANA_E_INVALID, "hana_batch_finalize: frames of a batch need distinct targets and shadow maps");
+    int r;
+    if ((r = check_status(c)) || (r = join(c))) return r;
+    batch_split(b);
+    for (auto& part : b->parts)
+        if ((r = batch_size_part(c, b, part))) return r;
+    if (b->parts.size() == 2 && !b->fork) {
+        HIP_TRY(c, hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+        HIP_TRY(c, hipEventCreateWithFlags(&b->joined, hipEventDisableTiming));
     }
     b->use_graph = use_graph != 0;
     if (b->use_graph) {
         HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int er = enqueue_sequence(c, b->plan, b->arena, c->stream, false);
+        int er = batch_enqueue(c, b, false);
         hipGraph_t g = nullptr;
         hipError_t ec = hipStreamEndCapture(c->stream, &g);
-        if (er) return er;
+        if (er) {
+            if (g) (void)hipGraphDestroy(g);
+            return er;
+        }
         if (ec != hipSuccess) return hip_fail(c, ec, "hipStreamEndCapture");
         b->graph = g;
         HIP_TRY(c, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
@@ -952,12 +1018,12 @@ int hana_batch_run(hana_ctx* c, hana_batch* b) {
     batch_drop_pending(b);
     int r = join(c);
     if (r) return r;
-    c->next_slot = 0;   // the next draw follows the sequence on the same stream
+    c->next_slot = 0;   // the next draw follows the sequence on the context stream
     if (b->use_graph) {
         HIP_TRY(c, hipGraphLaunch(b->exec, c->stream));
         return HANA_OK;
     }
-    return enqueue_sequence(c, b->plan, b->arena, c->stream);
+    return batch_enqueue(c, b, true);
 }
 
 int hana_batch_destroy(hana_ctx* c, hana_batch* b) {
@@ -965,7 +1031,9 @@ int hana_batch_destroy(hana_ctx* c, hana_batch* b) {
     (void)sync_all(c);
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    free_arena(b->arena);
+    for (auto& part : b->parts) free_arena(part.arena);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->joined) (void)hipEventDestroy(b->joined);
     delete b;
     return HANA_OK;
 }

@@ -98,6 +98,26 @@ def test_batch_without_graph(ctx, scenes, oracle_frames):
     batch.close()
 
 
+@pytest.mark.parametrize("pick", [[3], [0, 7, 11]])
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_small_batches(ctx, scenes, oracle_frames, pick, use_graph):
+    """One frame (a single part) and three frames (two uneven parts that run
+    concurrently), captured and eager, replayed twice."""
+    from hana_amd.gpu import Batch, SceneOnDevice
+    devs = [SceneOnDevice(ctx, scenes[i]) for i in pick]
+    batch = Batch(ctx)
+    for d in devs:
+        batch.add(d.desc())
+    batch.finalize(use_graph=use_graph)
+    for _ in range(2):
+        batch.run()
+        for i, (col, dep) in zip(pick, read_all(ctx, devs)):
+            tgt = oracle_frames[i][0]
+            assert_depth_exact(dep, tgt.depth, scenes[i].name)
+            assert_color_ok(col, tgt.color, scenes[i].name)
+    batch.close()
+
+
 def test_batch_rejects_shared_targets(ctx, scenes):
     from hana_amd.gpu import Batch, HanaError, SceneOnDevice
     d = SceneOnDevice(ctx, scenes[0])
