@@ -105,8 +105,9 @@ struct hana_ctx {
     hipStream_t stream2 = nullptr;
     Arena arena2;
     uint32_t* status_host2 = nullptr;
+    uint32_t* status_dev[2] = {};      // device addresses of status_host, status_host2
     hipEvent_t slot_done[2] = {};
-    bool slot_used[2] = {};
+    bool slot_pending[2] = {};         // a draw of slot k the other stream has not waited for
     int next_slot = 0;
     bool pipelined = true;
     // device staging for hana_target_present

@@ -776,6 +776,13 @@ __device__ __forceinline__ void load_counts(const uint32_t* c, int b, int n, uin
     }
 }
 
+// Set-up and clipping are complete when the scan runs: their counts become
+// the needs the host sizes the next launch sequence by.
+__device__ __forceinline__ void note_clip_needs(const Workspace& ws) {
+    atomicMax(&ws.status[STATUS_NEED_POLYS], ws.status[STATUS_POLYS]);
+    atomicMax(&ws.status[STATUS_NEED_TRIS], ws.status[STATUS_TRIS]);
+}
+
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(Workspace ws) {
     __shared__ uint32_t sh[SCAN_THREADS / 64];
     int n = ws.ntiles;
@@ -787,17 +794,17 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(Workspace ws) {
     for (int i = 0; i < SCAN_PER_THREAD; i++) s += v[i];
     uint32_t t = block_sum(s, sh);
     if (threadIdx.x == 0) ws.scan_part[blockIdx.x] = t;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {   // set-up and clipping are complete
-        atomicMax(&ws.status[STATUS_NEED_POLYS], ws.status[STATUS_POLYS]);
-        atomicMax(&ws.status[STATUS_NEED_TRIS], ws.status[STATUS_TRIS]);
-    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) note_clip_needs(ws);
 }
 
+// With a single chunk k_scan_reduce is not launched and this kernel alone
+// scans the counts.
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(Workspace ws) {
     __shared__ uint32_t sh[SCAN_THREADS / 64];
     __shared__ uint32_t wsum[SCAN_THREADS / 64];
     int n = ws.ntiles;
     int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (gridDim.x == 1 && tid == 0) note_clip_needs(ws);
     // sum of the chunks before this one
     uint32_t before = 0;
     for (int i = tid; i < (int)blockIdx.x; i += SCAN_THREADS) before += ws.scan_part[i];
@@ -846,6 +853,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(Workspace ws) {
         ws.status[STATUS_BINS_NEEDED] = run;
         atomicMax(&ws.status[STATUS_NEED_BINS], run);
         if (run > ws.bin_cap) atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_BIN_OVERFLOW);
+        // every status word is final now (binning only reads them)
+        if (ws.status_out)
+            for (int i = 0; i < STATUS_WORDS; i++)
+                ws.status_out[i] = __hip_atomic_load(&ws.status[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -1150,9 +1161,11 @@ hipError_t launch_clip(const PassDesc* d_passes, const Workspace& ws, hipStream_
 int scan_chunks(int ntiles) { return ntiles > 0 ? (ntiles + SCAN_CHUNK - 1) / SCAN_CHUNK : 1; }
 hipError_t launch_scan(const Workspace& ws, hipStream_t s) {
     int chunks = scan_chunks(ws.ntiles);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(chunks), dim3(SCAN_THREADS), 0, s, ws);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if (chunks > 1) {
+        hipLaunchKernelGGL(k_scan_reduce, dim3(chunks), dim3(SCAN_THREADS), 0, s, ws);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(k_scan_apply, dim3(chunks), dim3(SCAN_THREADS), 0, s, ws);
     return hipGetLastError();
 }

@@ -48,7 +48,10 @@ int sync_all(hana_ctx* c) {
 }
 
 int join(hana_ctx* c) {
-    if (c->slot_used[1]) HIP_TRY(c, hipStreamWaitEvent(c->stream, c->slot_done[1], 0));
+    if (c->slot_pending[1]) {
+        HIP_TRY(c, hipStreamWaitEvent(c->stream, c->slot_done[1], 0));
+        c->slot_pending[1] = false;
+    }
     return HANA_OK;
 }
 
@@ -245,6 +248,7 @@ int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_p
     ws.scan_part = A.scan_part.as<uint32_t>();
     ws.clipq_cap = A.clipq_cap;
     ws.status = A.status.as<uint32_t>();
+    ws.status_out = nullptr;
     ws.ntiles = pl.total_tiles;
     ws.bin_cap = A.bin_cap;
     ws.poly_cap = A.poly_cap;
@@ -451,10 +455,6 @@ int check_status(hana_ctx* c) {
 // calls in order.  With timing or statistics on, every call uses slot 0.
 int run_passes(hana_ctx* c, std::vector<PassSpec> specs) {
     int r;
-    int a = (c->pipelined && c->stream2 && !c->timing && !c->stats_enabled) ? c->next_slot : 0;
-    hipStream_t S = a ? c->stream2 : c->stream;
-    Arena& A = a ? c->arena2 : c->arena;
-    uint32_t* status_host = a ? c->status_host2 : c->status_host;
     std::vector<hana_target*> sampled;   // sampled as a shadow map, not drawn here
     for (auto& s : specs) {
         if (!s.shadow || !s.shadow->pending_clear) continue;
@@ -464,7 +464,11 @@ int run_passes(hana_ctx* c, std::vector<PassSpec> specs) {
     }
     Plan pl;
     if ((r = build_plan(c, std::move(specs), pl))) return r;
+    int a = (c->pipelined && c->stream2 && !c->timing && !c->stats_enabled) ? c->next_slot : 0;
+    hipStream_t S = a ? c->stream2 : c->stream;
+    Arena& A = a ? c->arena2 : c->arena;
     if ((r = bind_plan(c, pl, A, 0, 0, 0, c->stats_enabled, S))) return r;
+    pl.ws.status_out = c->status_dev[a];   // the scan leaves the status words in host memory
     // a target sampled as a shadow map by a pass of the same plan must be drawn
     // by an earlier group; clears fuse into the first pass writing the target
     take_clears(pl);
@@ -474,10 +478,10 @@ int run_passes(hana_ctx* c, std::vector<PassSpec> specs) {
         HIP_TRY(c, launch_fill32(reinterpret_cast<uint32_t*>(A.stats.p), 0u, (size_t)ST_WORDS * 2 * n, S));
     }
     if ((r = enqueue_geometry(c, pl, A, S, true))) return r;
-    // every status word is final after binning; copied before the wait so the
-    // copy stays off the path from one raster to the next
-    HIP_TRY(c, hipMemcpyAsync(status_host, A.status.p, sizeof(uint32_t) * STATUS_WORDS, hipMemcpyDeviceToHost, S));
-    if (c->slot_used[1 - a]) HIP_TRY(c, hipStreamWaitEvent(S, c->slot_done[1 - a], 0));
+    if (c->slot_pending[1 - a]) {
+        HIP_TRY(c, hipStreamWaitEvent(S, c->slot_done[1 - a], 0));
+        c->slot_pending[1 - a] = false;
+    }
     for (hana_target* t : sampled)
         if ((r = materialize_clear(c, t, S))) return r;
     if ((r = enqueue_raster(c, pl, A, S, true))) return r;
@@ -493,7 +497,7 @@ int run_passes(hana_ctx* c, std::vector<PassSpec> specs) {
                                   hipMemcpyDeviceToHost, S));
         c->stats_passes = n;
     }
-    c->slot_used[a] = true;
+    c->slot_pending[a] = true;
     c->next_slot = 1 - a;
     c->status_pending = true;
     return HANA_OK;
@@ -542,6 +546,9 @@ int hana_gpu_create(int device, hana_ctx** out) {
         if (hipEventCreateWithFlags(&c->slot_done[i], hipEventDisableTiming) != hipSuccess) return undo();
     if (hipHostMalloc((void**)&c->status_host2, sizeof(uint32_t) * STATUS_WORDS) != hipSuccess) return undo();
     std::memset(c->status_host2, 0, sizeof(uint32_t) * STATUS_WORDS);
+    if (hipHostGetDevicePointer((void**)&c->status_dev[0], c->status_host, 0) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&c->status_dev[1], c->status_host2, 0) != hipSuccess)
+        return undo();
     *out = c;
     return HANA_OK;
 }

@@ -120,6 +120,7 @@ struct Workspace {
     uint32_t* seg_count;         // records in each k_setup workgroup's segment
     uint32_t* scan_part;         // per scan chunk: sum of its tile counts
     uint32_t* status;            // see STATUS_*
+    uint32_t* status_out;        // or null: host-mapped copy of the status words, written by the scan
     int32_t ntiles;
     uint32_t bin_cap;
     uint32_t poly_cap;

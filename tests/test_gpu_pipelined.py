@@ -1,9 +1,10 @@
-"""Consecutive draws with no read in between.  The per-call path alternates
-two streams with their own workspaces, so a draw's geometry stages overlap
-the previous draw's raster; the results must still be those of the calls
-run one after another: depth tests across draws, shadow maps written by one
-call and sampled by the next, targets read, uploaded or presented between
-calls, and timing or statistics switched on in the middle of a chain."""
+"""Consecutive draws with no read in between, on small (150x110) and large
+(2048x2048) targets.  The per-call path alternates two streams with their
+own workspaces, so a draw's geometry stages overlap the previous draw's
+raster; the results must still be those of the calls run one after another:
+depth tests across draws, shadow maps written by one call and sampled by the
+next, targets read, uploaded or presented between calls, and timing switched
+on in the middle of a chain."""
 import numpy as np
 import pytest
 
@@ -13,7 +14,10 @@ from pyoracle import CPURenderer, HostTarget
 
 pytestmark = pytest.mark.gpu
 
-W, H = 150, 110
+
+@pytest.fixture(scope="module", params=[(150, 110), (2048, 2048)], ids=["small", "large"])
+def wh(request):
+    return request.param
 
 
 @pytest.fixture(scope="module")
@@ -29,25 +33,26 @@ def oracle():
     return CPURenderer("oracle")
 
 
-def chain_scenes(n, seed0):
+def chain_scenes(n, seed0, wh):
     kinds = [sg.NORMALMAP, sg.BLINN, sg.TEXTURE_LIGHT, sg.TOON, sg.GROUND, sg.TEXTURE]
     cams = [(0.0, 0.0, 2.0), (0.4, 0.3, 1.8), (-0.5, 0.1, 1.6), (0.2, -0.6, 2.2)]
-    return [sg.small_scene(kinds[i % len(kinds)], seed=seed0 + i, width=W, height=H, cam=cams[i % len(cams)])
+    return [sg.small_scene(kinds[i % len(kinds)], seed=seed0 + i, width=wh[0], height=wh[1], cam=cams[i % len(cams)])
             for i in range(n)]
 
 
 def oracle_chain(oracle, scenes, target=None):
-    t = target or HostTarget(W, H)
+    t = target or HostTarget(scenes[0].width, scenes[0].height)
     for sc in scenes:
         oracle.draw(sc.mesh, sc.shader, oracle.build_uniforms(sc), t, sc.diffuse, sc.normal)
     return t
 
 
 @pytest.mark.parametrize("n", [2, 3, 7])
-def test_draw_chain_into_one_target(ctx, oracle, n):
+def test_draw_chain_into_one_target(ctx, oracle, wh, n):
     """n draws of different meshes into one target, depth-tested against each other."""
     from hana_amd.gpu import build_uniforms
-    scenes = chain_scenes(n, 300 + n)
+    W, H = wh
+    scenes = chain_scenes(n, 300 + n, wh)
     tgt = ctx.target(W, H)
     keep = []
     for sc in scenes:
@@ -60,10 +65,11 @@ def test_draw_chain_into_one_target(ctx, oracle, n):
     assert_color_ok(col, ref.color, f"chain{n}")
 
 
-def test_alternating_frames_with_shadows(ctx, oracle):
+def test_alternating_frames_with_shadows(ctx, oracle, wh):
     """Two shadowed scenes rendered in turn, each frame drawing its shadow map
     and sampling it in the next call; read only at the end."""
     from hana_amd.gpu import SceneOnDevice
+    W, H = wh
     scenes = [sg.small_scene(k, shadow=True, seed=310 + k, width=W, height=H) for k in (sg.NORMALMAP, sg.BLINN)]
     devs = [SceneOnDevice(ctx, sc) for sc in scenes]
     for _ in range(3):
@@ -77,11 +83,12 @@ def test_alternating_frames_with_shadows(ctx, oracle):
         assert_color_ok(col, ref.color, sc.name)
 
 
-def test_reads_uploads_and_presents_between_draws(ctx, oracle):
+def test_reads_uploads_and_presents_between_draws(ctx, oracle, wh):
     """Target operations on the context stream are ordered after draws of
     either slot, and later draws after them."""
     from hana_amd.gpu import build_uniforms
-    scenes = chain_scenes(5, 320)
+    W, H = wh
+    scenes = chain_scenes(5, 320, wh)
     tgt = ctx.target(W, H)
     ref = HostTarget(W, H)
     keep = []
