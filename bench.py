@@ -427,6 +427,9 @@ def main():
                      "unit": "GB/s", "frac": round(dom_gbps / HBM_PEAK_GBPS, 5),
                      "bytes_per_launch": per_stage_bytes.get(dom, 0), "traffic": traffic,
                      "traffic_source": TRAFFIC_SOURCE if traffic is not None else None,
+                     "launch_us": round(stage_ms[dom] * 1e3, 2),
+                     "time_source": f"HIP events around each launch on the library stream, {args.steps} frames "
+                                    "with stage timing on (one stream, frames not overlapped)",
                      "frame_alg_bytes": frame_bytes,
                      "frame_achieved": round(frame_bytes / (ms_per_step * 1e-3) / 1e9, 2)},
         "parity": parity,
