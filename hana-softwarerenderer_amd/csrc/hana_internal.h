@@ -118,9 +118,10 @@ struct hana_ctx {
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
 };
 
-// A batch's frames are split into up to two parts, each a launch sequence
-// with its own workspace; the parts run concurrently on the context's two
-// streams (two branches of one graph), since their frames share no target.
+// A batch's frames are split into up to kBatchParts parts, each a launch
+// sequence with its own workspace; the parts run concurrently, part 0 on the
+// context stream, part 1 on its second stream and further parts on streams
+// of the batch (branches of one graph), since their frames share no target.
 struct hana_batch_part {
     std::vector<hana::rt::PassSpec> specs;
     hana::rt::Plan plan;
@@ -135,7 +136,9 @@ struct hana_batch {
     bool use_graph = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipEvent_t fork = nullptr, joined = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> joined;    // one per part after the first
+    std::vector<hipStream_t> extra;    // streams of parts 2, 3, ...
 };
 
 namespace hana {

@@ -172,9 +172,10 @@ int hana_draw(hana_ctx* ctx, const hana_draw_desc* d);
 int hana_render_frame(hana_ctx* ctx, const hana_frame_desc* f);
 
 /* ---- batches: many frames in one captured graph ----
- * The frames are split into two parts that run concurrently (on the context's
- * two streams, two branches of the graph); they need distinct targets and
- * shadow maps.  hana_batch_run returns once the work is enqueued. */
+ * The frames are split into up to four parts that run concurrently (branches
+ * of the graph on the context's streams and the batch's own); they need
+ * distinct targets and shadow maps.  hana_batch_run returns once the work is
+ * enqueued. */
 int hana_batch_create(hana_ctx* ctx, hana_batch** out);
 int hana_batch_add_frame(hana_ctx* ctx, hana_batch* b, const hana_frame_desc* f);
 /* Build the workspace and (when use_graph) capture the launch sequence. */
