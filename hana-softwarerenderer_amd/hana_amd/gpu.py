@@ -125,6 +125,11 @@ DECLS = [
     ("hana_server_destroy", C.c_int, [_P]),
     ("hana_build_uniforms", C.c_int, [_FP, _FP, C.c_float, _FP, _FP, _FP, _FP, C.c_int32, C.c_int32, C.c_int32,
                                       C.POINTER(Uniforms)]),
+    ("hana_scene_init", C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    ("hana_scene_update_camera", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("hana_scene_key", C.c_int, [C.c_void_p, C.c_int32]),
+    ("hana_scene_shader", C.c_int, [C.c_void_p]),
+    ("hana_scene_uniforms", C.c_int, [C.c_void_p, _FP, _FP, _FP, C.c_int32, C.c_int32, C.POINTER(Uniforms)]),
     ("hana_asset_last_error", C.c_char_p, []),
     ("hana_obj_load", C.c_int, [C.c_char_p, C.POINTER(ObjMesh)]),
     ("hana_obj_free", C.c_int, [C.POINTER(ObjMesh)]),
@@ -167,6 +172,75 @@ def _f32(a, n=None):
 
 def _fp(a):
     return a.ctypes.data_as(_FP)
+
+
+class Motion(C.Structure):
+    """hana_motion: Motion (SR/camera.h:11), deltas in window heights."""
+    _fields_ = [("orbit", C.c_float * 2), ("pan", C.c_float * 2), ("dolly", C.c_float)]
+
+
+class SceneStateS(C.Structure):
+    _fields_ = [("camera_position", C.c_float * 3), ("camera_target", C.c_float * 3), ("aspect", C.c_float),
+                ("light_position", C.c_float * 3), ("shader_index", C.c_int32), ("enable_shadow", C.c_int32)]
+
+
+KEY_A, KEY_D, KEY_Q, KEY_E, KEY_SPACE = 0, 1, 4, 5, 6
+
+
+class InteractiveScene:
+    """The state SingleModelScene keeps between frames (hana_scene_*): the
+    orbit camera, the light, the shader and shadow switches.  frame() gives
+    the Scene to render for the current state."""
+
+    def __init__(self, width: int, height: int):
+        self.lib = load_library()
+        self.width, self.height = int(width), int(height)
+        self.s = SceneStateS()
+        self._check(self.lib.hana_scene_init(C.byref(self.s), self.width, self.height), "init")
+
+    def _check(self, rc, what):
+        if rc < 0:
+            raise HanaError(rc, f"hana_scene_{what}")
+        return rc
+
+    def move(self, orbit=(0.0, 0.0), pan=(0.0, 0.0), dolly=0.0):
+        m = Motion((C.c_float * 2)(*orbit), (C.c_float * 2)(*pan), float(dolly))
+        self._check(self.lib.hana_scene_update_camera(C.byref(self.s), C.byref(m)), "update_camera")
+
+    def key(self, k: int):
+        self._check(self.lib.hana_scene_key(C.byref(self.s), int(k)), "key")
+
+    @property
+    def shader(self) -> int:
+        return self._check(self.lib.hana_scene_shader(C.byref(self.s)), "shader")
+
+    @property
+    def shadow(self) -> bool:
+        return bool(self.s.enable_shadow)
+
+    @property
+    def camera_position(self) -> tuple:
+        return tuple(self.s.camera_position)
+
+    @property
+    def camera_target(self) -> tuple:
+        return tuple(self.s.camera_target)
+
+    @property
+    def light_position(self) -> tuple:
+        return tuple(self.s.light_position)
+
+    def uniforms(self, obj_position=(0.0, 0.0, 0.0), obj_rotation=(0.0, 0.0, 0.0), obj_scale=(1.0, 1.0, 1.0)):
+        u = Uniforms()
+        arrs = [_f32(x) for x in (obj_position, obj_rotation, obj_scale)]
+        self._check(self.lib.hana_scene_uniforms(C.byref(self.s), _fp(arrs[0]), _fp(arrs[1]), _fp(arrs[2]),
+                                                 self.width, self.height, C.byref(u)), "uniforms")
+        return u
+
+    def frame(self, mesh, diffuse, normal, name="interactive"):
+        from . import scenegen as sg
+        return sg.Scene(mesh, self.width, self.height, self.shader, self.shadow, self.camera_position,
+                        self.light_position, diffuse, normal, cam_target=self.camera_target, name=name)
 
 
 def build_uniforms(scene, enable_shadow=None) -> Uniforms:

@@ -19,6 +19,8 @@
  *   hana_obj_load        Model::Model OBJ parser + corner fetches   SR/model.cpp:6-48,66-111
  *   hana_tga_load/save   TGAImage::read_tga_file / write_tga_file   SR/tgaimage.cpp:40-246
  *   HANA_SHADER_*        the IShader subclasses                     SR/IShader.h:132-165
+ *   hana_scene_*         Scene / SingleModelScene state, orbit      SR/scene.cpp:3-9,45-63,127-147,
+ *                        camera and its reset                       SR/camera.cpp:31-70,167-169
  *
  * A context owns two HIP streams, each with its own workspace, and every
  * device object created on it; calls on one context must be externally
@@ -280,6 +282,47 @@ int hana_tga_load(const char* path, int flip_vertically, hana_image* out);
  * RLE packets as the reference forms them, TGA 2.0 footer; byte-identical. */
 int hana_tga_save(const char* path, const hana_image* img, int rle);
 int hana_image_free(hana_image* img);
+
+/* ---- interactive scene state ----
+ * The state the reference's main loop keeps between frames: the orbit camera,
+ * the light, the shader and shadow switches.  A sequence of inputs applied
+ * here, then hana_scene_uniforms + hana_render_frame per frame, renders what
+ * SingleModelScene::tick draws for the same inputs. */
+typedef struct hana_motion {   /* Motion (SR/camera.h:11): deltas in window heights */
+    float orbit[2];
+    float pan[2];
+    float dolly;
+} hana_motion;
+
+typedef struct hana_scene {
+    float camera_position[3];
+    float camera_target[3];   /* moved by panning; lights and shadows use the origin (SR/camera.cpp:94-97) */
+    float aspect;
+    float light_position[3];
+    int32_t shader_index;     /* 0 NormalMap, 1 Blinn, 2 Texture+Light, 3 Texture, 4 Toon, 5 Ground (SR/scene.cpp:86-97) */
+    int32_t enable_shadow;
+} hana_scene;
+
+enum {   /* the keys the scene reacts to (SR/platform.h:7) */
+    HANA_KEY_A = 0,      /* light rotates -5 degrees about (0,1,1) */
+    HANA_KEY_D = 1,      /* light rotates +5 degrees */
+    HANA_KEY_E = 5,      /* shadow on/off */
+    HANA_KEY_Q = 4,      /* next shader; shadow on for the first three */
+    HANA_KEY_SPACE = 6   /* camera back to (0,0,2) -> origin */
+};
+
+/* Scene::Scene for a width x height framebuffer: camera (0,0,2) -> origin,
+ * light (2,2,2), shadow on, NormalMap. */
+int hana_scene_init(hana_scene* s, int32_t width, int32_t height);
+/* Camera::update_transform (pan, then orbit and dolly about the target). */
+int hana_scene_update_camera(hana_scene* s, const hana_motion* m);
+/* Scene::on_key_input / SingleModelScene::on_key_input for a pressed key. */
+int hana_scene_key(hana_scene* s, int32_t key);
+/* The HANA_SHADER_* kind of the scene's current shader. */
+int hana_scene_shader(const hana_scene* s);
+/* hana_build_uniforms for the scene's camera, light and shadow switch. */
+int hana_scene_uniforms(const hana_scene* s, const float obj_position[3], const float obj_rotation_deg[3],
+                        const float obj_scale[3], int32_t fb_w, int32_t fb_h, hana_uniforms* out);
 
 /* ---- host uniform math ---- */
 /* ShaderData as DrawModel::draw fills it for Camera(cam_pos, cam_target,

@@ -161,6 +161,13 @@ class CPURenderer:
         self._frame = sig("render_frame", C.c_int, C.POINTER(MeshS), C.c_int, C.c_int, C.POINTER(Uniforms),
                           C.POINTER(Image), C.POINTER(Image), C.POINTER(Target), C.POINTER(Target),
                           C.POINTER(Stats), C.POINTER(Stats))
+        self._camera_update = getattr(L, "ref_camera_update", None)
+        if self._camera_update is not None:
+            self._camera_update.restype = None
+            self._camera_update.argtypes = [fp, fp, C.c_float, fp, fp, fp]
+            self._scene_keys = L.ref_scene_keys
+            self._scene_keys.restype = C.c_int
+            self._scene_keys.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int, fp]
         self._draw_model = getattr(L, "ref_draw_model", None)
         if self._draw_model is not None:
             self._draw_model.restype = C.c_int
@@ -280,6 +287,30 @@ class CPURenderer:
     def is_back_facing(self, ndc9):
         a, p = self._f(ndc9)
         return bool(self._backface(p))
+
+    def camera_update(self, pos, target, aspect, motion):
+        """Camera::update_transform (reference library only): new (position, target)."""
+        if self._camera_update is None:
+            raise RuntimeError(f"{self.path} has no camera entry point")
+        fp = C.POINTER(C.c_float)
+        a = [np.ascontiguousarray(x, np.float32) for x in (pos, target, motion)]
+        op, ot = np.zeros(3, np.float32), np.zeros(3, np.float32)
+        self._camera_update(*(x.ctypes.data_as(fp) for x in a[:2]), np.float32(aspect), a[2].ctypes.data_as(fp),
+                            op.ctypes.data_as(fp), ot.ctypes.data_as(fp))
+        return op, ot
+
+    def scene_keys(self, obj_path, width, height, keys):
+        """SingleModelScene driven by pressed keys (reference library only):
+        per key (light x, y, z, enable_shadow, shader index)."""
+        if self._camera_update is None:
+            raise RuntimeError(f"{self.path} has no scene entry point")
+        k = (C.c_int * len(keys))(*keys)
+        out = np.zeros((len(keys), 5), np.float32)
+        rc = self._scene_keys(str(obj_path).encode(), width, height, k, len(keys),
+                              out.ctypes.data_as(C.POINTER(C.c_float)))
+        if rc:
+            raise RuntimeError("ref_scene_keys: shader line not recognised")
+        return out
 
     def build_uniforms(self, scene, enable_shadow=None) -> Uniforms:
         u = Uniforms()

@@ -17,9 +17,14 @@
 //    rasterize_triangle, with 30 slots on both sides.  ref_draw_model calls the
 //    unmodified graphics_draw_triangle on a Model read from an OBJ file, for
 //    scenes without clipped triangles.
-//  * camera.cpp needs the Win32 platform layer, so the camera matrices are built
-//    here from the reference's lookat/perspective with the constants of
-//    SR/camera.cpp:11-14, as Camera::get_view_matrix/get_proj_matrix do.
+//  * ref_build_uniforms builds the camera matrices from the reference's
+//    lookat/perspective with the constants of SR/camera.cpp:11-14, as
+//    Camera::get_view_matrix/get_proj_matrix do.  camera.cpp and scene.cpp are
+//    compiled too, for the Camera and Scene state entry points; camera.cpp's
+//    input callbacks (get_cursor_pos, scroll_callback, button_callback,
+//    update_camera) call the Win32 platform layer, are referenced by nothing
+//    here, and are dropped at link time (oracle/ref.map exports only the ref_
+//    entry points; --gc-sections).
 //  * DrawModel::draw (SR/scene.h:53-99) is restated in ref_build_uniforms and
 //    ref_render_frame with the reference's types and functions.
 //  * Built with -DHANA_DROPIN (oracle/_ref/libhanaref_dropin.so), the
@@ -48,7 +53,9 @@ void graphics_draw_triangle(DrawData* draw_data);
 #include "tgaimage.cpp"
 #include "model.cpp"
 #include "gameobject.cpp"
-#include "scene.h"      // AMBIENT, LightColor (SR/scene.h:8-9); DrawModel is not instantiated
+#include "scene.h"      // AMBIENT, LightColor (SR/scene.h:8-9)
+#include "camera.cpp"
+#include "scene.cpp"
 #ifdef HANA_DROPIN
 #include "graphics_hana.cpp"
 #endif
@@ -428,6 +435,58 @@ int ref_render_frame(const ho_mesh* mesh, int kind, int enable_shadow, const ho_
         // RGB and depth are the reference's (SR/scene.h:96-97).
         const float black[4] = {0, 0, 0, 1};
         ref_target_clear(shadow_map, black, std::numeric_limits<float>::max());
+    }
+    return 0;
+}
+
+// Camera::update_transform (SR/camera.cpp:63-70) on Camera(pos, target, aspect)
+// for Motion {orbit, pan, dolly} = motion[0..1], [2..3], [4].  Camera keeps
+// its target private with no accessor (get_target_position returns the
+// origin), so position and target are read through the class's layout.
+void ref_camera_update(const float* pos, const float* target, float aspect, const float* motion, float* out_pos,
+                       float* out_target) {
+    Camera cam(v3(pos), v3(target), aspect);
+    Motion m;
+    m.orbit = Vector2f(motion[0], motion[1]);
+    m.pan = Vector2f(motion[2], motion[3]);
+    m.dolly = motion[4];
+    cam.update_transform(m);
+    struct Layout {
+        Vector3f position, target;
+        float aspect;
+    };
+    static_assert(sizeof(Layout) == sizeof(Camera), "Camera holds position, target, aspect");
+    Layout l;
+    memcpy(&l, &cam, sizeof l);
+    out_pos[0] = l.position.x; out_pos[1] = l.position.y; out_pos[2] = l.position.z;
+    out_target[0] = l.target.x; out_target[1] = l.target.y; out_target[2] = l.target.z;
+}
+
+// SingleModelScene(obj_path) over a w x h RenderBuffer, driven by n pressed
+// keys (SR/scene.cpp:45-63,127-147).  After each key the scene ticks (it draws
+// its frame) and out[5*i..] receives the light position, enable_shadow and the
+// shader index, read from the "shader: " line tick() writes (SR/scene.cpp:119).
+// Returns 0, or -1 when a shader line is not recognised.
+int ref_scene_keys(const char* obj_path, int w, int h, const int* keys, int n, float* out) {
+    static const char* des[6] = {"6.NormalMap", "5.Blinn", "4.Texture_Light(no shadow)", "5.Texture(no shadow)",
+                                 "2.Toon(no shadow)", "1.Ground(no shadow)"};   // SR/scene.cpp:86-97
+    RenderBuffer rb(w, h);
+    SingleModelScene scene(obj_path, &rb);
+    for (int i = 0; i < n; i++) {
+        scene.on_key_input((keycode_t)keys[i], 1);
+        scene.tick(0.0f);
+        const char* line = strstr(scene.get_text(), "shader: ");
+        int idx = -1;
+        for (int k = 0; line && k < 6 && idx < 0; k++) {
+            size_t len = strlen(des[k]);
+            if (strncmp(line + 8, des[k], len) == 0 && line[8 + len] == '\n') idx = k;
+        }
+        if (idx < 0) return -1;
+        Vector3f lp = scene.light->transform.position;
+        float* o = out + 5 * i;
+        o[0] = lp.x; o[1] = lp.y; o[2] = lp.z;
+        o[3] = scene.enable_shadow ? 1.0f : 0.0f;
+        o[4] = (float)idx;
     }
     return 0;
 }
