@@ -72,6 +72,7 @@ struct Plan {
     size_t total_v = 0, total_vs = 0, total_f = 0;
     size_t uni_off = 0;   // byte offset of the uniforms in the tables buffer
     int total_tiles = 0;
+    bool small_tris = true;   // every pass has at most 4 target pixels per face
     Workspace ws{};
 };
 

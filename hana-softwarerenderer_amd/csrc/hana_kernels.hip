@@ -647,14 +647,18 @@ __device__ __forceinline__ void bin_record(const PassDesc* passes, const Workspa
 // The tiles of one workgroup's (triangle, tile) pairs in an LDS hash table:
 // open addressing over BIN_HASH slots, a bounded probe; a tile that finds no
 // slot goes through the global cursors instead.
-constexpr int BIN_HASH = 2048;
+// Two table sizes: 512 slots for sequences of small triangles (a segment of
+// 1024 faces then touches a few dozen tiles, and a smaller table is quicker
+// to clear and scan), 2048 for larger ones.
 constexpr int BIN_PROBES = 32;
 constexpr uint32_t BIN_EMPTY = 0xFFFFFFFFu;
 
+template <int BITS>
 __device__ __forceinline__ int bin_slot(uint32_t* keys, uint32_t gt, bool insert) {
-    uint32_t h = (gt * 2654435761u) >> (32 - 11);
+    constexpr uint32_t SLOTS = 1u << BITS;
+    uint32_t h = (gt * 2654435761u) >> (32 - BITS);
     for (int p = 0; p < BIN_PROBES; p++) {
-        uint32_t sl = (h + (uint32_t)p) & (BIN_HASH - 1);
+        uint32_t sl = (h + (uint32_t)p) & (SLOTS - 1);
         uint32_t k = keys[sl];
         if (k == gt) return (int)sl;
         if (k == BIN_EMPTY) {
@@ -671,7 +675,9 @@ __device__ __forceinline__ int bin_slot(uint32_t* keys, uint32_t gt, bool insert
 // bin with one global atomic, then every pair takes a place in that range
 // with an LDS atomic.  The other workgroups stride over the clipped fan
 // triangles listed after the segments.
+template <int BITS>
 __global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* passes, Workspace ws, int nseg) {
+    constexpr int BIN_HASH = 1 << BITS;
     if ((int)blockIdx.x < nseg) {
         __shared__ uint32_t keys[BIN_HASH];
         __shared__ uint32_t cnt[BIN_HASH];
@@ -697,7 +703,7 @@ __global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* passes, Workspace
             for (int ty = ty0; ty <= ty1; ty++)
                 for (int tx = tx0; tx <= tx1; tx++) {
                     uint32_t gt = (uint32_t)(P.tile0 + ty * P.tiles_x + tx);
-                    int sl = bin_slot(keys, gt, true);
+                    int sl = bin_slot<BITS>(keys, gt, true);
                     if (sl >= 0) atomicAdd(&cnt[sl], 1u);
                 }
         }
@@ -714,7 +720,7 @@ __global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* passes, Workspace
             for (int ty = ty0; ty <= ty1; ty++)
                 for (int tx = tx0; tx <= tx1; tx++) {
                     uint32_t gt = (uint32_t)(P.tile0 + ty * P.tiles_x + tx);
-                    int sl = bin_slot(keys, gt, false);
+                    int sl = bin_slot<BITS>(keys, gt, false);
                     uint32_t pos = sl >= 0 ? atomicAdd(&cnt[sl], 1u) : atomicAdd(&ws.tile_cursor[gt], 1u);
                     if (pos < ws.bin_cap) ws.bins[pos] = r.x;
                 }
@@ -1169,10 +1175,13 @@ hipError_t launch_scan(const Workspace& ws, hipStream_t s) {
     hipLaunchKernelGGL(k_scan_apply, dim3(chunks), dim3(SCAN_THREADS), 0, s, ws);
     return hipGetLastError();
 }
-hipError_t launch_bin(const PassDesc* d_passes, const Workspace& ws, int nseg, hipStream_t s) {
+hipError_t launch_bin(const PassDesc* d_passes, const Workspace& ws, int nseg, bool small_tris, hipStream_t s) {
     // one workgroup per setup segment, then a grid-stride tail for the
     // clipped fan triangles (their count is only known on the device)
-    hipLaunchKernelGGL(k_bin, dim3(nseg + 512), dim3(BLOCK), 0, s, d_passes, ws, nseg);
+    if (small_tris)
+        hipLaunchKernelGGL(k_bin<9>, dim3(nseg + 512), dim3(BLOCK), 0, s, d_passes, ws, nseg);
+    else
+        hipLaunchKernelGGL(k_bin<11>, dim3(nseg + 512), dim3(BLOCK), 0, s, d_passes, ws, nseg);
     return hipGetLastError();
 }
 hipError_t launch_raster(const PassDesc* d_passes, int npasses, const Workspace& ws, int blocks, hipStream_t s) {

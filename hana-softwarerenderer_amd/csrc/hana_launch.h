@@ -11,7 +11,8 @@ hipError_t launch_setup(const PassDesc* d_passes, int npasses, const Workspace& 
 hipError_t launch_clip(const PassDesc* d_passes, const Workspace& ws, hipStream_t s);
 int scan_chunks(int ntiles);
 hipError_t launch_scan(const Workspace& ws, hipStream_t s);
-hipError_t launch_bin(const PassDesc* d_passes, const Workspace& ws, int nseg, hipStream_t s);
+// small_tris: every pass averages at most 4 pixels per face (smaller tile table)
+hipError_t launch_bin(const PassDesc* d_passes, const Workspace& ws, int nseg, bool small_tris, hipStream_t s);
 hipError_t launch_raster(const PassDesc* d_passes, int npasses, const Workspace& ws, int blocks, hipStream_t s);
 hipError_t launch_fill32(uint32_t* dst, uint32_t value, size_t n, hipStream_t s);
 hipError_t launch_present(const uint32_t* src, uint32_t* dst, int w, int h, hipStream_t s);

@@ -101,6 +101,7 @@ int build_plan(hana_ctx* c, std::vector<PassSpec> specs, Plan& pl) {
     size_t voff = 0, vsoff = 0, fo = 0;
     int vb = 0, fb = 0, tile0 = 0;
     int tb[2] = {0, 0};
+    pl.small_tris = true;
     for (int i = 0; i < n; i++) {
         const PassSpec& s = specs[i];
         if (s.group < 0 || s.group > 1) return fail(c, HANA_E_INVALID, "bad pass group");
@@ -126,6 +127,7 @@ int build_plan(hana_ctx* c, std::vector<PassSpec> specs, Plan& pl) {
         d.width = t->w; d.height = t->h;
         d.tiles_x = (t->w + TILE - 1) / TILE;
         d.tiles_y = (t->h + TILE - 1) / TILE;
+        if ((int64_t)t->w * t->h > 4 * (int64_t)m->F) pl.small_tris = false;
         const size_t vchunk = (size_t)BLOCK * VPT, fchunk = (size_t)BLOCK * FPT;
         size_t vpad = (V + vchunk - 1) / vchunk * vchunk;
         // slices: pointers are resolved by bind_plan once the arena exists
@@ -326,7 +328,7 @@ int enqueue_geometry(hana_ctx* c, const Plan& pl, const Arena& A, hipStream_t s,
     }
     {
         StageTimer t(c, 4, timed, s);
-        HIP_TRY(c, launch_bin(d, pl.ws, pl.fblocks, s));
+        HIP_TRY(c, launch_bin(d, pl.ws, pl.fblocks, pl.small_tris, s));
         t.done(s);
     }
     return HANA_OK;
