@@ -339,15 +339,54 @@ __device__ __forceinline__ void wave_count(uint32_t* counters, int t, bool want)
     }
 }
 
+// The tiles of one workgroup's (triangle, tile) pairs in an LDS hash table:
+// open addressing over BIN_HASH slots, a bounded probe; a tile that finds no
+// slot goes through the global cursors instead.
+// k_setup and k_bin size their tables by the sequence: 512 slots when every
+// pass has small triangles (a segment of 1024 faces then touches a few dozen
+// tiles, and a smaller table is quicker to clear and scan), more otherwise.
+constexpr int BIN_PROBES = 32;
+constexpr uint32_t BIN_EMPTY = 0xFFFFFFFFu;
+
+template <int BITS>
+__device__ __forceinline__ int bin_slot(uint32_t* keys, uint32_t gt, bool insert) {
+    constexpr uint32_t SLOTS = 1u << BITS;
+    uint32_t h = (gt * 2654435761u) >> (32 - BITS);
+    for (int p = 0; p < BIN_PROBES; p++) {
+        uint32_t sl = (h + (uint32_t)p) & (SLOTS - 1);
+        uint32_t k = keys[sl];
+        if (k == gt) return (int)sl;
+        if (k == BIN_EMPTY) {
+            if (!insert) return -1;
+            uint32_t old = atomicCAS(&keys[sl], BIN_EMPTY, gt);
+            if (old == BIN_EMPTY || old == gt) return (int)sl;
+        }
+    }
+    return -1;
+}
+
+// Counts one (triangle, tile) pair in the workgroup's LDS table, or in the
+// global counter when the table has no slot left for the tile.
+template <int BITS>
+__device__ __forceinline__ void lds_count(uint32_t* keys, uint32_t* cnt, uint32_t* global, uint32_t gt) {
+    int sl = bin_slot<BITS>(keys, gt, true);
+    if (sl >= 0) atomicAdd(&cnt[sl], 1u);
+    else atomicAdd(&global[gt], 1u);
+}
+
 // Triangles over more tiles than this are counted and binned by a whole wave
 // (or workgroup), their tiles spread over the lanes.
 constexpr int WIDE_TILES = 16;
 
 // Counts every (triangle, tile) pair of one lane's triangle and writes its
 // record (bin entry, pass, first and last tile) for k_bin to ws.tris[slot];
-// wave-uniform.
+// wave-uniform.  With BITS > 0 the pairs of triangles over at most WIDE_TILES
+// tiles go to the workgroup's LDS table (keys, cnt), which the caller adds to
+// the global counts; with BITS == 0 they go to the global counts directly,
+// one atomic per distinct tile of the wave.
+template <int BITS>
 __device__ __forceinline__ void count_tiles(const PassDesc& P, const Workspace& ws, int pass, bool ok, const TriS& t,
-                                            uint32_t entry, uint32_t slot) {
+                                            uint32_t entry, uint32_t slot, uint32_t* keys, uint32_t* cnt) {
     int tx0 = 0, ty0 = 0, tx1 = 0, ty1 = 0, nx = 0, n = 0;
     if (ok) {
         tx0 = t.x0 >> TILE_SHIFT;
@@ -371,10 +410,15 @@ __device__ __forceinline__ void count_tiles(const PassDesc& P, const Workspace& 
             atomicAdd(&ws.tile_count[wt0 + (wy0 + k / wnx) * wtx + wx0 + k % wnx], 1u);
     }
     int ns = n > WIDE_TILES ? 0 : n;
-    for (int k = 0; __ballot(k < ns); k++) {
-        bool want = k < ns;
-        int gt = want ? tile0 + (ty0 + k / nx) * tiles_x + tx0 + k % nx : 0;
-        wave_count(ws.tile_count, gt, want);
+    if constexpr (BITS == 0) {
+        for (int k = 0; __ballot(k < ns); k++) {
+            bool want = k < ns;
+            int gt = want ? tile0 + (ty0 + k / nx) * tiles_x + tx0 + k % nx : 0;
+            wave_count(ws.tile_count, gt, want);
+        }
+    } else {
+        for (int k = 0; k < ns; k++)
+            lds_count<BITS>(keys, cnt, ws.tile_count, (uint32_t)(tile0 + (ty0 + k / nx) * tiles_x + tx0 + k % nx));
     }
     if (P.stats) {
         for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off);
@@ -411,10 +455,17 @@ __device__ __forceinline__ bool dies_at_first_crossed_plane(uint32_t c0, uint32_
 // (BLOCK * FPT records from blockIdx.x * BLOCK * FPT; the count goes to
 // ws.seg_count); faces that clipping would remove are dropped; the others are
 // queued for k_clip with one global atomic per workgroup.
+template <int BITS>
 __global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* passes, int npasses, Workspace ws) {
+    constexpr int SLOTS = 1 << BITS;
     __shared__ uint64_t s_queue[BLOCK * FPT];
+    __shared__ uint32_t s_keys[SLOTS], s_cnt[SLOTS];   // tile counts of the workgroup's triangles
     __shared__ uint32_t s_ntri, s_nq, s_qbase;
     if (threadIdx.x == 0) { s_ntri = 0; s_nq = 0; }
+    for (int i = threadIdx.x; i < SLOTS; i += BLOCK) {
+        s_keys[i] = BIN_EMPTY;
+        s_cnt[i] = 0u;
+    }
     __syncthreads();
     int pass = find_pass<1>(passes, npasses, blockIdx.x);
     const PassDesc& P = passes[pass];
@@ -467,9 +518,11 @@ __global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* passes, int npa
         TriS t;
         bool ok = trivial[k] && tri_setup(va[k], vb[k], vc[k], P.width, P.height, t);
         uint32_t slot = wave_reserve_lds(&s_ntri, ok);
-        count_tiles(P, ws, pass, ok, t, (uint32_t)f, seg + slot);
+        count_tiles<BITS>(P, ws, pass, ok, t, (uint32_t)f, seg + slot, s_keys, s_cnt);
     }
     __syncthreads();
+    for (int i = threadIdx.x; i < SLOTS; i += BLOCK)
+        if (s_keys[i] != BIN_EMPTY) atomicAdd(&ws.tile_count[s_keys[i]], s_cnt[i]);
     if (threadIdx.x == 0) {
         ws.seg_count[blockIdx.x] = s_ntri;
         s_qbase = s_nq ? atomicAdd(&ws.status[STATUS_CLIPQ], s_nq) : 0u;
@@ -606,7 +659,8 @@ __global__ __launch_bounds__(CLIP_THREADS) void k_clip(const PassDesc* passes, W
             }
             P.poly_of_face[f] = keep ? (int)pid : -1;
         }
-        count_tiles(P, ws, pass, ok, t, CLIPPED_BIT | (pid << 3) | (uint32_t)li, ws.seg_region + slot);
+        count_tiles<0>(P, ws, pass, ok, t, CLIPPED_BIT | (pid << 3) | (uint32_t)li, ws.seg_region + slot, nullptr,
+                       nullptr);
         __syncthreads();
     }
 }
@@ -642,32 +696,6 @@ __device__ __forceinline__ void bin_record(const PassDesc* passes, const Workspa
         uint32_t pos = wave_reserve(ws.tile_cursor, gt, want);
         if (want && pos < ws.bin_cap) ws.bins[pos] = r.x;
     }
-}
-
-// The tiles of one workgroup's (triangle, tile) pairs in an LDS hash table:
-// open addressing over BIN_HASH slots, a bounded probe; a tile that finds no
-// slot goes through the global cursors instead.
-// Two table sizes: 512 slots for sequences of small triangles (a segment of
-// 1024 faces then touches a few dozen tiles, and a smaller table is quicker
-// to clear and scan), 2048 for larger ones.
-constexpr int BIN_PROBES = 32;
-constexpr uint32_t BIN_EMPTY = 0xFFFFFFFFu;
-
-template <int BITS>
-__device__ __forceinline__ int bin_slot(uint32_t* keys, uint32_t gt, bool insert) {
-    constexpr uint32_t SLOTS = 1u << BITS;
-    uint32_t h = (gt * 2654435761u) >> (32 - BITS);
-    for (int p = 0; p < BIN_PROBES; p++) {
-        uint32_t sl = (h + (uint32_t)p) & (SLOTS - 1);
-        uint32_t k = keys[sl];
-        if (k == gt) return (int)sl;
-        if (k == BIN_EMPTY) {
-            if (!insert) return -1;
-            uint32_t old = atomicCAS(&keys[sl], BIN_EMPTY, gt);
-            if (old == BIN_EMPTY || old == gt) return (int)sl;
-        }
-    }
-    return -1;
 }
 
 // Workgroups below nseg scatter the setup segment of their index: the pairs
@@ -1154,9 +1182,13 @@ hipError_t launch_vertex(const PassDesc* d_passes, int npasses, const Workspace&
     hipLaunchKernelGGL(k_vertex, dim3(blocks), dim3(BLOCK), 0, s, d_passes, npasses, ws);
     return hipGetLastError();
 }
-hipError_t launch_setup(const PassDesc* d_passes, int npasses, const Workspace& ws, int blocks, hipStream_t s) {
+hipError_t launch_setup(const PassDesc* d_passes, int npasses, const Workspace& ws, int blocks, bool small_tris,
+                        hipStream_t s) {
     if (blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_setup, dim3(blocks), dim3(BLOCK), 0, s, d_passes, npasses, ws);
+    if (small_tris)
+        hipLaunchKernelGGL(k_setup<9>, dim3(blocks), dim3(BLOCK), 0, s, d_passes, npasses, ws);
+    else
+        hipLaunchKernelGGL(k_setup<10>, dim3(blocks), dim3(BLOCK), 0, s, d_passes, npasses, ws);
     return hipGetLastError();
 }
 hipError_t launch_clip(const PassDesc* d_passes, const Workspace& ws, hipStream_t s) {

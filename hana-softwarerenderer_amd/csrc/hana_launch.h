@@ -7,7 +7,8 @@
 namespace hana {
 
 hipError_t launch_vertex(const PassDesc* d_passes, int npasses, const Workspace& ws, int blocks, hipStream_t s);
-hipError_t launch_setup(const PassDesc* d_passes, int npasses, const Workspace& ws, int blocks, hipStream_t s);
+hipError_t launch_setup(const PassDesc* d_passes, int npasses, const Workspace& ws, int blocks, bool small_tris,
+                        hipStream_t s);
 hipError_t launch_clip(const PassDesc* d_passes, const Workspace& ws, hipStream_t s);
 int scan_chunks(int ntiles);
 hipError_t launch_scan(const Workspace& ws, hipStream_t s);

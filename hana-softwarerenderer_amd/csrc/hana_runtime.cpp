@@ -313,7 +313,7 @@ int enqueue_geometry(hana_ctx* c, const Plan& pl, const Arena& A, hipStream_t s,
     }
     {
         StageTimer t(c, 1, timed, s);
-        HIP_TRY(c, launch_setup(d, n, pl.ws, pl.fblocks, s));
+        HIP_TRY(c, launch_setup(d, n, pl.ws, pl.fblocks, pl.small_tris, s));
         t.done(s);
     }
     {
