@@ -56,6 +56,7 @@ struct Arena {
     // tables: the pass descriptors, then (at Plan::uni_off) their uniforms,
     // uploaded by one copy per call
     DevBuf tables, vr, vs, code, pof, tile_count, tile_offset, tile_cursor, bins, tris, seg_count, scan_part;
+    DevBuf seg_tiles, seg_wide, seg_info;
     DevBuf pool_vr, pool_vs, pool_face, pool_pass, pool_n, clip_queue, status, stats;
     uint32_t bin_cap = 0, poly_cap = 0, clipq_cap = 0, tris_cap = 0;
 };

@@ -384,9 +384,16 @@ constexpr int WIDE_TILES = 16;
 // tiles go to the workgroup's LDS table (keys, cnt), which the caller adds to
 // the global counts; with BITS == 0 they go to the global counts directly,
 // one atomic per distinct tile of the wave.
+struct SegTables {       // k_setup's LDS tables of one segment
+    uint32_t* keys;
+    uint32_t* cnt;
+    uint16_t* wide;      // segment-local indices of the triangles over more than WIDE_TILES tiles
+    uint32_t* nwide;
+};
+
 template <int BITS>
 __device__ __forceinline__ void count_tiles(const PassDesc& P, const Workspace& ws, int pass, bool ok, const TriS& t,
-                                            uint32_t entry, uint32_t slot, uint32_t* keys, uint32_t* cnt) {
+                                            uint32_t entry, uint32_t slot, uint32_t local, const SegTables* L) {
     int tx0 = 0, ty0 = 0, tx1 = 0, ty1 = 0, nx = 0, n = 0;
     if (ok) {
         tx0 = t.x0 >> TILE_SHIFT;
@@ -400,6 +407,9 @@ __device__ __forceinline__ void count_tiles(const PassDesc& P, const Workspace& 
     }
     // a triangle over more than WIDE_TILES tiles is counted by the whole wave
     int tile0 = P.tile0, tiles_x = P.tiles_x;
+    if constexpr (BITS > 0) {
+        if (n > WIDE_TILES) L->wide[atomicAdd(L->nwide, 1u)] = (uint16_t)local;
+    }
     uint64_t wide = __ballot(n > WIDE_TILES);
     while (wide) {
         int src = __ffsll((unsigned long long)wide) - 1;
@@ -418,7 +428,7 @@ __device__ __forceinline__ void count_tiles(const PassDesc& P, const Workspace& 
         }
     } else {
         for (int k = 0; k < ns; k++)
-            lds_count<BITS>(keys, cnt, ws.tile_count, (uint32_t)(tile0 + (ty0 + k / nx) * tiles_x + tx0 + k % nx));
+            lds_count<BITS>(L->keys, L->cnt, ws.tile_count, (uint32_t)(tile0 + (ty0 + k / nx) * tiles_x + tx0 + k % nx));
     }
     if (P.stats) {
         for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off);
@@ -460,8 +470,10 @@ __global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* passes, int npa
     constexpr int SLOTS = 1 << BITS;
     __shared__ uint64_t s_queue[BLOCK * FPT];
     __shared__ uint32_t s_keys[SLOTS], s_cnt[SLOTS];   // tile counts of the workgroup's triangles
-    __shared__ uint32_t s_ntri, s_nq, s_qbase;
-    if (threadIdx.x == 0) { s_ntri = 0; s_nq = 0; }
+    __shared__ uint16_t s_wide[BLOCK * FPT];
+    __shared__ uint32_t s_ntri, s_nq, s_qbase, s_nwide, s_nlist;
+    if (threadIdx.x == 0) { s_ntri = 0; s_nq = 0; s_nwide = 0; s_nlist = 0; }
+    const SegTables L{s_keys, s_cnt, s_wide, &s_nwide};
     for (int i = threadIdx.x; i < SLOTS; i += BLOCK) {
         s_keys[i] = BIN_EMPTY;
         s_cnt[i] = 0u;
@@ -518,16 +530,25 @@ __global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* passes, int npa
         TriS t;
         bool ok = trivial[k] && tri_setup(va[k], vb[k], vc[k], P.width, P.height, t);
         uint32_t slot = wave_reserve_lds(&s_ntri, ok);
-        count_tiles<BITS>(P, ws, pass, ok, t, (uint32_t)f, seg + slot, s_keys, s_cnt);
+        count_tiles<BITS>(P, ws, pass, ok, t, (uint32_t)f, seg + slot, slot, &L);
     }
     __syncthreads();
+    // add the segment's counts to the global ones, and hand k_bin the list of
+    // its tiles with their counts and the list of its wide triangles
+    uint2* list = ws.seg_tiles + (size_t)blockIdx.x * SLOTS;
     for (int i = threadIdx.x; i < SLOTS; i += BLOCK)
-        if (s_keys[i] != BIN_EMPTY) atomicAdd(&ws.tile_count[s_keys[i]], s_cnt[i]);
+        if (s_keys[i] != BIN_EMPTY) {
+            atomicAdd(&ws.tile_count[s_keys[i]], s_cnt[i]);
+            list[atomicAdd(&s_nlist, 1u)] = make_uint2(s_keys[i], s_cnt[i]);
+        }
+    uint32_t nwide = s_nwide;
+    for (uint32_t i = threadIdx.x; i < nwide; i += BLOCK) ws.seg_wide[(size_t)blockIdx.x * (BLOCK * FPT) + i] = s_wide[i];
     if (threadIdx.x == 0) {
         ws.seg_count[blockIdx.x] = s_ntri;
         s_qbase = s_nq ? atomicAdd(&ws.status[STATUS_CLIPQ], s_nq) : 0u;
     }
     __syncthreads();
+    if (threadIdx.x == 0) ws.seg_info[blockIdx.x] = s_nlist | (nwide << 16);
     for (uint32_t i = threadIdx.x; i < s_nq; i += BLOCK) {
         uint32_t q = s_qbase + i;
         if (q < ws.clipq_cap) ws.clip_queue[q] = s_queue[i];
@@ -659,7 +680,7 @@ __global__ __launch_bounds__(CLIP_THREADS) void k_clip(const PassDesc* passes, W
             }
             P.poly_of_face[f] = keep ? (int)pid : -1;
         }
-        count_tiles<0>(P, ws, pass, ok, t, CLIPPED_BIT | (pid << 3) | (uint32_t)li, ws.seg_region + slot, nullptr,
+        count_tiles<0>(P, ws, pass, ok, t, CLIPPED_BIT | (pid << 3) | (uint32_t)li, ws.seg_region + slot, 0u,
                        nullptr);
         __syncthreads();
     }
@@ -698,42 +719,34 @@ __device__ __forceinline__ void bin_record(const PassDesc* passes, const Workspa
     }
 }
 
-// Workgroups below nseg scatter the setup segment of their index: the pairs
-// are counted per tile in LDS, each distinct tile reserves its range of the
-// bin with one global atomic, then every pair takes a place in that range
-// with an LDS atomic.  The other workgroups stride over the clipped fan
-// triangles listed after the segments.
+// Workgroups below nseg scatter the setup segment of their index: the
+// segment's per-tile pair counts (listed by k_setup) go into an LDS table,
+// each distinct tile reserves its range of the bin with one global atomic,
+// then every pair takes a place in that range with an LDS atomic.  The other
+// workgroups stride over the clipped fan triangles listed after the segments.
 template <int BITS>
 __global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* passes, Workspace ws, int nseg) {
     constexpr int BIN_HASH = 1 << BITS;
     if ((int)blockIdx.x < nseg) {
         __shared__ uint32_t keys[BIN_HASH];
         __shared__ uint32_t cnt[BIN_HASH];
-        __shared__ uint16_t wide[BLOCK * FPT];
-        __shared__ uint32_t nwide;
         for (int i = threadIdx.x; i < BIN_HASH; i += BLOCK) {
             keys[i] = BIN_EMPTY;
             cnt[i] = 0u;
         }
-        if (threadIdx.x == 0) nwide = 0;
         __syncthreads();
         uint32_t n = ws.seg_count[blockIdx.x];
         const uint4* rec = ws.tris + (size_t)blockIdx.x * (BLOCK * FPT);
-        for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
-            uint4 r = rec[i];
-            const PassDesc& P = passes[r.y];
-            int tx0 = (int)(r.z & 0xFFFFu), ty0 = (int)(r.z >> 16);
-            int tx1 = (int)(r.w & 0xFFFFu), ty1 = (int)(r.w >> 16);
-            if ((tx1 - tx0 + 1) * (ty1 - ty0 + 1) > WIDE_TILES) {   // binned by the workgroup below
-                wide[atomicAdd(&nwide, 1u)] = (uint16_t)i;
-                continue;
-            }
-            for (int ty = ty0; ty <= ty1; ty++)
-                for (int tx = tx0; tx <= tx1; tx++) {
-                    uint32_t gt = (uint32_t)(P.tile0 + ty * P.tiles_x + tx);
-                    int sl = bin_slot<BITS>(keys, gt, true);
-                    if (sl >= 0) atomicAdd(&cnt[sl], 1u);
-                }
+        // the segment's distinct tiles and their pair counts, as k_setup listed
+        // them (a tile that found no slot there, or finds none here, goes
+        // through the global cursor pair by pair below)
+        uint32_t info = ws.seg_info[blockIdx.x];
+        uint32_t nlist = info & 0xFFFFu, nw = info >> 16;
+        const uint2* list = ws.seg_tiles + (size_t)blockIdx.x * ws.seg_stride;
+        for (uint32_t i = threadIdx.x; i < nlist; i += BLOCK) {
+            uint2 e = list[i];
+            int sl = bin_slot<BITS>(keys, e.x, true);
+            if (sl >= 0) cnt[sl] = e.y;   // listed tiles are distinct: the slot is this thread's
         }
         __syncthreads();
         for (int i = threadIdx.x; i < BIN_HASH; i += BLOCK)
@@ -754,7 +767,7 @@ __global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* passes, Workspace
                 }
         }
         // wide triangles, one at a time, their tiles spread over the workgroup
-        uint32_t nw = nwide;
+        const uint16_t* wide = ws.seg_wide + (size_t)blockIdx.x * (BLOCK * FPT);
         for (uint32_t w = 0; w < nw; w++) {
             uint4 r = rec[wide[w]];
             const PassDesc& P = passes[r.y];

@@ -211,6 +211,12 @@ int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_p
         A.tris_cap = (uint32_t)want_tris;
     }
     if ((r = ensure(c, A.seg_count, sizeof(uint32_t) * std::max(pl.fblocks, 1)))) return r;
+    // k_setup's per-segment tile lists, as long as its LDS table (launch_setup)
+    uint32_t seg_stride = pl.small_tris ? 512u : 1024u;
+    size_t nsegs = (size_t)std::max(pl.fblocks, 1);
+    if ((r = ensure(c, A.seg_tiles, sizeof(uint2) * seg_stride * nsegs))) return r;
+    if ((r = ensure(c, A.seg_wide, sizeof(uint16_t) * BLOCK * FPT * nsegs))) return r;
+    if ((r = ensure(c, A.seg_info, sizeof(uint32_t) * nsegs))) return r;
     // every face may fail the trivial accept (camera inside the mesh)
     uint64_t want_q = std::max<uint64_t>(pl.total_f, 1024);
     want_q = std::min<uint64_t>(want_q, 0xFFFFFFF0ull);
@@ -247,6 +253,10 @@ int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_p
     ws.tris_cap = A.tris_cap;
     ws.seg_region = (uint32_t)seg_region;
     ws.seg_count = A.seg_count.as<uint32_t>();
+    ws.seg_tiles = A.seg_tiles.as<uint2>();
+    ws.seg_wide = A.seg_wide.as<uint16_t>();
+    ws.seg_info = A.seg_info.as<uint32_t>();
+    ws.seg_stride = seg_stride;
     ws.scan_part = A.scan_part.as<uint32_t>();
     ws.clipq_cap = A.clipq_cap;
     ws.status = A.status.as<uint32_t>();
@@ -556,7 +566,7 @@ int hana_gpu_create(int device, hana_ctx** out) {
 }
 
 static void free_arena(Arena& a) {
-    DevBuf* bufs[] = {&a.tables, &a.vr, &a.vs, &a.code, &a.tris, &a.seg_count, &a.scan_part, &a.pof, &a.tile_count, &a.tile_offset,
+    DevBuf* bufs[] = {&a.tables, &a.vr, &a.vs, &a.code, &a.tris, &a.seg_count, &a.seg_tiles, &a.seg_wide, &a.seg_info, &a.scan_part, &a.pof, &a.tile_count, &a.tile_offset,
                       &a.tile_cursor, &a.bins, &a.pool_vr, &a.pool_vs, &a.pool_face, &a.pool_pass, &a.pool_n,
                       &a.clip_queue, &a.status, &a.stats};
     for (DevBuf* b : bufs)

@@ -118,6 +118,11 @@ struct Workspace {
     uint4* tris;                 // every triangle with tiles: bin entry, pass, first tile, last tile;
                                  //   BLOCK*FPT per k_setup workgroup, then the clipped fan triangles
     uint32_t* seg_count;         // records in each k_setup workgroup's segment
+    uint2* seg_tiles;            // per segment, seg_stride entries: (tile, pairs) of its triangles
+                                 //   over at most WIDE_TILES tiles, counted in k_setup's LDS table
+    uint16_t* seg_wide;          // per segment, BLOCK*FPT entries: its triangles over more tiles
+    uint32_t* seg_info;          // per segment: listed tiles | wide triangles << 16
+    uint32_t seg_stride;
     uint32_t* scan_part;         // per scan chunk: sum of its tile counts
     uint32_t* status;            // see STATUS_*
     uint32_t* status_out;        // or null: host-mapped copy of the status words, written by the scan
