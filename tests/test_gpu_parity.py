@@ -154,6 +154,54 @@ def test_tile_scan_large_targets(ctx, oracle, wh):
     check_scene(ctx, oracle, scene)
 
 
+def soup_scene(n, width, height, px_lo, px_hi, seed):
+    """n front-facing triangles of px_lo..px_hi pixels across, scattered in
+    random order over the view of the z = 0 plane from (0, 0, 2)."""
+    rng = np.random.default_rng(seed)
+    half = 2.0 * np.tan(np.radians(30.0))                  # visible half-height at z = 0
+    aspect = width / height
+    centre = rng.uniform(-0.95, 0.95, (n, 2)) * (half * aspect, half)
+    size = half * 2 / height * rng.uniform(px_lo, px_hi, n) / 2
+    ang = rng.uniform(0, 2 * np.pi, n)
+    corners = np.stack([ang, ang + 2.1, ang + 4.2], 1)     # counter-clockwise: front-facing
+    xy = centre[:, None, :] + size[:, None, None] * np.stack([np.cos(corners), np.sin(corners)], 2)
+    z = np.repeat(rng.uniform(-0.3, 0.3, (n, 1, 1)), 3, 1)
+    pos = np.concatenate([xy, z], 2).reshape(-1, 3).astype(np.float32)
+    nrm = sg.fixed_point_normals(np.tile(np.array([[0.0, 0.0, 1.0]], np.float32), (3 * n, 1)))
+    uv = rng.uniform(0, 1, (3 * n, 2)).astype(np.float32)
+    mesh = sg.Mesh(pos, nrm, uv, np.arange(3 * n, dtype=np.int32).reshape(n, 3))
+    base = sg.small_scene(sg.BLINN, width=width, height=height, seed=seed)
+    return sg.Scene(mesh, width, height, sg.BLINN, False, (0.0, 0.0, 2.0), base.light_pos, base.diffuse,
+                    base.normal, name=f"soup{n}")
+
+
+@pytest.mark.parametrize("n,wh,px", [(70_000, (512, 512), (1.5, 4.0)), (60_000, (2048, 2048), (8.0, 40.0))],
+                         ids=["small-tris", "wide"])
+def test_scattered_triangle_soup(ctx, oracle, n, wh, px):
+    """Triangles in random order, so each segment of 1024 faces touches more
+    distinct tiles than the tile tables of set-up and binning hold (512 slots
+    for sequences of small triangles, at most 4 target pixels per face; 1024
+    and 2048 otherwise): those tiles fall back to the global counters and
+    cursors, and the frame still equals the oracle's."""
+    scene = soup_scene(n, wh[0], wh[1], px[0], px[1], seed=2024 + n)
+    # the premise: most segments list more distinct tiles than the tables hold
+    w, h = wh
+    p = scene.mesh.pos[scene.mesh.idx]                     # (F, 3, 3); eye at z = 2, 60 degree fovy
+    k = 1.0 / (np.tan(np.radians(30.0)) * (2.0 - p[..., 2]))
+    sx = (p[..., 0] * k / (w / h) + 1) * w / 2
+    sy = (p[..., 1] * k + 1) * h / 2
+    tx0, tx1 = (np.clip([sx.min(1), sx.max(1)], 0, w - 1) // 16).astype(int)
+    ty0, ty1 = (np.clip([sy.min(1), sy.max(1)], 0, h - 1) // 16).astype(int)
+    slots = 512 if w * h <= 4 * n else 2048
+    over = 0
+    for s0 in range(0, n - 1023, 1024):
+        tiles = {(y, x) for f in range(s0, s0 + 1024)
+                 for y in range(ty0[f], ty1[f] + 1) for x in range(tx0[f], tx1[f] + 1)}
+        over += len(tiles) > slots
+    assert over > n // 1024 // 2
+    check_scene(ctx, oracle, scene)
+
+
 @pytest.mark.slow
 def test_config2_full(ctx, oracle):
     scene = sg.config2_scene()
