@@ -198,6 +198,7 @@ __global__ __launch_bounds__(BLOCK) void k_vertex(const PassDesc* passes, int np
     const DevUniforms& u = *P.u;
     int base = (blockIdx.x - P.vblock0) * BLOCK * VPT;
     bool need_n = P.kind != SHADER_SHADOW && P.kind != SHADER_TEXTURE;
+    bool need_uv = P.kind != SHADER_SHADOW && P.kind != SHADER_GROUND && P.kind != SHADER_TOON;
     // all loads of the thread's VPT vertices first (coalesced SoA streams)
     F3 pos[VPT], nrm[VPT];
     float tu[VPT], tv[VPT];
@@ -210,8 +211,10 @@ __global__ __launch_bounds__(BLOCK) void k_vertex(const PassDesc* passes, int np
         if (v < P.nverts) {
             pos[k] = f3(P.px[v], P.py[v], P.pz[v]);
             if (need_n) nrm[k] = f3(P.nx[v], P.ny[v], P.nz[v]);
-            tu[k] = P.tu[v];
-            tv[k] = P.tv[v];
+            if (need_uv) {
+                tu[k] = P.tu[v];
+                tv[k] = P.tv[v];
+            }
         }
     }
 #pragma unroll
