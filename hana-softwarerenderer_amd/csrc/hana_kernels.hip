@@ -1090,7 +1090,10 @@ __device__ __forceinline__ void shade_pixel(const PassDesc& P, const Workspace& 
     size_t pix = (size_t)py * P.width + px;
     uint32_t alpha = P.clear ? P.clear_rgba : P.color[pix];
     P.color[pix] = pack_rgb(col, alpha);
-    P.depth[pix] = z;
+    // A NaN depth is stored as the x86 default NaN (quiet, sign set): that is
+    // what the reference's own arithmetic produces for it (an invalid
+    // operation such as inf/inf yields that pattern and NaN operands pass it on).
+    P.depth[pix] = (z != z) ? __uint_as_float(0xFFC00000u) : z;
     if (P.winner) P.winner[pix] = order;
 }
 

@@ -1,0 +1,52 @@
+"""Scenes for edge cases of the depth test, shared by the oracle and GPU tests."""
+import numpy as np
+
+from hana_amd import scenegen as sg
+
+
+def _quad(x0, y0, x1, y1, pz):
+    p = np.array([[x0, y0, pz], [x1, y0, pz], [x1, y1, pz], [x0, y1, pz]], np.float32)
+    return p, np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+
+
+# Clip transform of the NaN-depth scene: x = s*px, y = s*py, z = s*pz - s,
+# w = s*pz with s = 1e10, so ndc = (px/pz, py/pz, 1 - 1/pz).  A vertex at
+# pz = 1e30 overflows z and w to +inf: it passes the trivial accept
+# (|x|, |y|, |z| <= w), lands on the centre of the screen and its depth
+# (inf/inf + 1) * 0.5 is NaN, so every fragment of its triangles has NaN depth.
+NAN_SCENE_VP = np.zeros((4, 4), np.float32)
+NAN_SCENE_VP[0, 0] = NAN_SCENE_VP[1, 1] = NAN_SCENE_VP[2, 2] = NAN_SCENE_VP[3, 2] = np.float32(1e10)
+NAN_SCENE_VP[2, 3] = -np.float32(1e10)
+
+
+def nan_depth_scene(width=96, height=72, seed=3, shift=0.0):
+    """A near quad, then a triangle of NaN depth over it, then two farther
+    quads: the NaN fragments replace the near quad's depth (NaN > d is false),
+    and every fragment after the last NaN one at a pixel passes.  `shift`
+    moves the NaN triangle's far corners, for a second draw into the same
+    target."""
+    parts = [_quad(-0.8, -0.8, 0.8, 0.8, 2.0),
+             (np.array([[0.3, 0.2, 1e30], [-1.6 + shift, -1.4, 2.0], [1.5, -1.2 + shift, 2.0]], np.float32),
+              np.array([[0, 1, 2]], np.int32)),
+             _quad(-0.6, -1.0, 1.0, 0.5, 3.0),
+             _quad(-1.0, -0.2, 0.4, 1.0, 2.5)]
+    pos, idx, n = [], [], 0
+    for p, i in parts:
+        pos.append(p)
+        idx.append(i + n)
+        n += len(p)
+    pos = np.concatenate(pos)
+    rng = np.random.default_rng(seed)
+    uv = rng.uniform(0, 1, (len(pos), 2)).astype(np.float32)
+    nrm = np.tile(np.array([[0.0, 0.0, 1.0]], np.float32), (len(pos), 1))
+    mesh = sg.Mesh(pos, nrm, uv, np.concatenate(idx))
+    base = sg.small_scene(sg.TEXTURE, width=width, height=height, seed=seed)
+    return sg.Scene(mesh, width, height, sg.TEXTURE, False, (0.0, 0.0, 2.0), base.light_pos, base.diffuse,
+                    base.normal, name=f"nan{shift}")
+
+
+def with_camera_vp(u, m=NAN_SCENE_VP):
+    """The uniforms u with camera_vp replaced by m (row-major)."""
+    for i, v in enumerate(np.asarray(m, np.float32).ravel()):
+        u.camera_vp[i] = v
+    return u

@@ -7,6 +7,7 @@ interpolated light-space depth) bit-exact.
 import numpy as np
 import pytest
 
+from edge_scenes import nan_depth_scene, with_camera_vp
 from hana_amd import scenegen as sg
 from parity import assert_color_ok, assert_depth_exact
 from pyoracle import CPURenderer, HostTarget
@@ -200,6 +201,32 @@ def test_scattered_triangle_soup(ctx, oracle, n, wh, px):
         over += len(tiles) > slots
     assert over > n // 1024 // 2
     check_scene(ctx, oracle, scene)
+
+
+def test_nan_depth_fragments(ctx, oracle):
+    """Fragments of NaN depth (a vertex whose clip z and w overflow to inf):
+    they replace any stored depth, every later fragment at their pixels
+    passes, and a second draw into the same target starts from the stored
+    NaNs.  Depth (its NaN bit pattern included), winners and colour equal
+    the oracle's."""
+    from hana_amd.gpu import build_uniforms
+    W, H = 96, 72
+    tgt = ctx.target(W, H, winner=True)
+    ref = HostTarget(W, H)
+    keep = []
+    for shift in (0.0, 0.3):
+        sc = nan_depth_scene(W, H, shift=shift)
+        u = with_camera_vp(build_uniforms(sc))
+        keep.append(u)
+        ctx.draw(ctx.mesh(sc.mesh), sc.shader, u, tgt, ctx.texture(sc.diffuse), ctx.texture(sc.normal))
+        _, owin = oracle.draw(sc.mesh, sc.shader, with_camera_vp(oracle.build_uniforms(sc)), ref, sc.diffuse,
+                              sc.normal, winner=True)
+    col, dep = ctx.read(tgt, W, H)
+    win = ctx.read_winner(tgt, W, H)
+    assert np.isnan(ref.depth).sum() > 300
+    assert_depth_exact(dep, ref.depth, "nan depth")
+    assert np.array_equal(win.astype(np.int64), as_u32_ids(owin)), "winners"
+    assert_color_ok(col, ref.color, "nan depth")
 
 
 @pytest.mark.slow

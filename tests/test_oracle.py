@@ -12,6 +12,7 @@ import os
 import numpy as np
 import pytest
 
+from edge_scenes import nan_depth_scene, with_camera_vp
 from hana_amd import scenegen as sg
 from pyoracle import REF_SO, CPURenderer, HostTarget, Uniforms
 
@@ -146,6 +147,22 @@ def test_live_frames_match_reference(oracle, ref, seed):
     if sc.shadow:
         assert np.array_equal(a[1].color[..., :3], b[1].color[..., :3])
         assert beq(a[1].depth, b[1].depth)
+
+
+def test_live_nan_depth_matches_reference(oracle, ref):
+    """Fragments of NaN depth replace the stored depth and let every later
+    fragment pass, within a draw and into the next one: the oracle's depth
+    (NaN bit patterns included) and colour equal the reference's."""
+    out = []
+    for r in (oracle, ref):
+        t = HostTarget(96, 72)
+        for shift in (0.0, 0.3):
+            sc = nan_depth_scene(shift=shift)
+            r.draw(sc.mesh, sc.shader, with_camera_vp(r.build_uniforms(sc)), t, sc.diffuse, sc.normal)
+        out.append(t)
+    assert np.isnan(out[0].depth).sum() > 300
+    assert beq(out[0].depth, out[1].depth)
+    assert np.array_equal(out[0].color[..., :3], out[1].color[..., :3])
 
 
 def test_live_config1_matches_reference(oracle, ref):
