@@ -18,6 +18,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 sys.path[:0] = [os.path.join(REPO, "oracle"), os.path.join(REPO, "hana-softwarerenderer_amd")]
 
+sys.path.insert(0, os.path.dirname(HERE))
+
+from edge_scenes import NAN_SCENE_VP, nan_depth_scene, with_camera_vp  # noqa: E402
 from hana_amd import scenegen as sg  # noqa: E402
 from pyoracle import CPURenderer, HostTarget  # noqa: E402
 
@@ -172,17 +175,39 @@ def frame_fixtures(ref):
     return out
 
 
-def main():
+def edge_fixtures(ref):
+    """Two draws of the NaN-depth scene into one target: the mesh and camera
+    transform of each draw, and the target after both."""
+    out = {"camera_vp": NAN_SCENE_VP}
+    t = HostTarget(96, 72)
+    for i, shift in enumerate((0.0, 0.3)):
+        sc = nan_depth_scene(shift=shift)
+        ref.draw(sc.mesh, sc.shader, with_camera_vp(ref.build_uniforms(sc)), t, sc.diffuse, sc.normal)
+        for k, v in {"pos": sc.mesh.pos, "nrm": sc.mesh.nrm, "uv": sc.mesh.uv, "idx": sc.mesh.idx,
+                     "diffuse": sc.diffuse, "light": np.array(sc.light_pos, np.float32)}.items():
+            out[f"nan{i}__{k}"] = v
+    out["nan__color"], out["nan__depth"] = t.color, t.depth
+    return out
+
+
+def main(which=None):
+    """Writes every fixture file, or only those named (without .npz)."""
     ref = CPURenderer("reference")
     rng = np.random.default_rng(SEED)
-    np.savez_compressed(os.path.join(HERE, "kat_math.npz"), **math_kats(ref, rng), **uniform_kats(ref))
-    np.savez_compressed(os.path.join(HERE, "kat_raster.npz"), **raster_kats(ref, rng))
-    np.savez_compressed(os.path.join(HERE, "kat_shaders.npz"), **shader_kats(ref, rng))
-    np.savez_compressed(os.path.join(HERE, "frames.npz"), **frame_fixtures(ref))
+    files = {
+        "kat_math": lambda: {**math_kats(ref, rng), **uniform_kats(ref)},
+        "kat_raster": lambda: raster_kats(ref, rng),
+        "kat_shaders": lambda: shader_kats(ref, rng),
+        "frames": lambda: frame_fixtures(ref),
+        "edge_frames": lambda: edge_fixtures(ref),
+    }
+    for name, make in files.items():
+        if not which or name in which:
+            np.savez_compressed(os.path.join(HERE, name + ".npz"), **make())
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

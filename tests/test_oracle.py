@@ -130,6 +130,21 @@ def test_golden_frames(oracle, name):
         assert st_s["fragments"] == int(k("shadow_fragments"))
 
 
+def test_golden_nan_depth(oracle):
+    """Two draws with NaN-depth fragments into one target, as the reference
+    rendered them (edge_frames.npz): depth bits, NaNs included, and colour."""
+    g = load("edge_frames.npz")
+    t = HostTarget(96, 72)
+    for i in range(2):
+        k = lambda s: g[f"nan{i}__{s}"]
+        sc = sg.Scene(sg.Mesh(k("pos"), k("nrm"), k("uv"), k("idx")), 96, 72, sg.TEXTURE, False, (0.0, 0.0, 2.0),
+                      tuple(float(x) for x in k("light")), k("diffuse"), k("diffuse"))
+        oracle.draw(sc.mesh, sc.shader, with_camera_vp(oracle.build_uniforms(sc), g["camera_vp"]), t, sc.diffuse)
+    assert np.isnan(t.depth).sum() > 300
+    assert beq(t.depth, g["nan__depth"])
+    assert np.array_equal(t.color[..., :3], g["nan__color"][..., :3])
+
+
 # ---------------------------------------------------------------- live reference
 
 @pytest.mark.parametrize("seed", range(6))
