@@ -50,3 +50,28 @@ def with_camera_vp(u, m=NAN_SCENE_VP):
     for i, v in enumerate(np.asarray(m, np.float32).ravel()):
         u.camera_vp[i] = v
     return u
+
+
+def sliver_scene(n=800_000, width=2048, height=2048, seed=11):
+    """n slivers 1 pixel wide and 32 tall, each straddling a tile column
+    boundary and spanning three tile rows: six (triangle, tile) pairs per
+    triangle, more than the initial bin capacity of four per face, with random
+    depths.  With camera_vp the identity, positions are NDC."""
+    rng = np.random.default_rng(seed)
+    tx = rng.integers(1, width // 16, n)
+    ty = rng.integers(0, height // 16 - 2, n)
+    sx = tx * 16 - 0.5 + rng.uniform(-0.2, 0.2, n)
+    sy = ty * 16 + 8 + rng.uniform(-2.0, 2.0, n)
+    corners = np.stack([np.stack([sx, sy], 1), np.stack([sx + 1.0, sy + 2.0], 1), np.stack([sx + 0.5, sy + 32.0], 1)], 1)
+    ndc = corners / np.array([width, height]) * 2 - 1
+    z = np.repeat(rng.uniform(-0.9, 0.9, (n, 1, 1)), 3, 1)
+    pos = np.concatenate([ndc, z], 2).reshape(-1, 3).astype(np.float32)
+    uv = rng.uniform(0, 1, (3 * n, 2)).astype(np.float32)
+    nrm = np.tile(np.array([[0.0, 0.0, 1.0]], np.float32), (3 * n, 1))
+    mesh = sg.Mesh(pos, nrm, uv, np.arange(3 * n, dtype=np.int32).reshape(n, 3))
+    base = sg.small_scene(sg.TEXTURE, width=256, height=256, seed=seed)
+    return sg.Scene(mesh, width, height, sg.TEXTURE, False, (0.0, 0.0, 2.0), base.light_pos, base.diffuse,
+                    base.normal, name="slivers")
+
+
+IDENTITY_VP = np.eye(4, dtype=np.float32)

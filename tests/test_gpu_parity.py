@@ -7,7 +7,7 @@ interpolated light-space depth) bit-exact.
 import numpy as np
 import pytest
 
-from edge_scenes import nan_depth_scene, with_camera_vp
+from edge_scenes import IDENTITY_VP, nan_depth_scene, sliver_scene, with_camera_vp
 from hana_amd import scenegen as sg
 from parity import assert_color_ok, assert_depth_exact
 from pyoracle import CPURenderer, HostTarget
@@ -295,6 +295,37 @@ def check_regrowth(ctx, oracle):
     assert np.array_equal(win.astype(np.int64), as_u32_ids(owin))
     assert_depth_exact(dep, ref.depth, "straddling")
     assert_color_ok(col, ref.color, "straddling")
+
+
+def test_bin_overflow_regrows(oracle):
+    """800 k slivers of six (triangle, tile) pairs each overflow the bins the
+    first draw allocates (four pairs per face): the read reports
+    HANA_E_OVERFLOW, the bins have grown, and the re-render is exact.  A
+    context of its own, so that no earlier test has grown the bins."""
+    from hana_amd.gpu import HANA_E_OVERFLOW, Context, HanaError, build_uniforms
+    scene = sliver_scene()
+    W, H = scene.width, scene.height
+    with Context(0) as ctx:
+        u = with_camera_vp(build_uniforms(scene), IDENTITY_VP)
+        mesh, dif, nrm = ctx.mesh(scene.mesh), ctx.texture(scene.diffuse), ctx.texture(scene.normal)
+        tgt = ctx.target(W, H, winner=True)
+        ctx.clear(tgt)
+        ctx.draw(mesh, scene.shader, u, tgt, dif, nrm)
+        with pytest.raises(HanaError) as err:
+            ctx.read(tgt, W, H)
+        assert err.value.code == HANA_E_OVERFLOW
+        for _ in range(2):                                   # both draw slots have grown
+            ctx.clear(tgt)
+            ctx.draw(mesh, scene.shader, u, tgt, dif, nrm)
+            col, dep = ctx.read(tgt, W, H)
+            win = ctx.read_winner(tgt, W, H)
+        ref = HostTarget(W, H)
+        st, owin = oracle.draw(scene.mesh, scene.shader, with_camera_vp(oracle.build_uniforms(scene), IDENTITY_VP),
+                               ref, scene.diffuse, scene.normal, winner=True)
+        assert st["fragments"] > 1_000_000
+        assert np.array_equal(win.astype(np.int64), as_u32_ids(owin))
+        assert_depth_exact(dep, ref.depth, "slivers")
+        assert_color_ok(col, ref.color, "slivers")
 
 
 def test_clip_pool_overflow_in_a_batch(ctx, oracle):
