@@ -1197,7 +1197,10 @@ __global__ void k_present(const uint32_t* __restrict__ src, uint32_t* __restrict
 // ---------------------------------------------------------------------------
 
 hipError_t launch_vertex(const PassDesc* d_passes, int npasses, const Workspace& ws, int blocks, hipStream_t s) {
-    if (blocks <= 0) return hipSuccess;
+    if (npasses <= 0) return hipSuccess;
+    // k_vertex also zeroes the tile counters and the status words of the
+    // sequence, so it runs even when no pass has a vertex
+    blocks = std::max(blocks, 1);
     hipLaunchKernelGGL(k_vertex, dim3(blocks), dim3(BLOCK), 0, s, d_passes, npasses, ws);
     return hipGetLastError();
 }

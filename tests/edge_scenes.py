@@ -75,3 +75,14 @@ def sliver_scene(n=800_000, width=2048, height=2048, seed=11):
 
 
 IDENTITY_VP = np.eye(4, dtype=np.float32)
+
+
+def empty_mesh():
+    """A mesh with no vertices and no faces."""
+    return sg.Mesh(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.float32), np.zeros((0, 2), np.float32),
+                   np.zeros((0, 3), np.int32))
+
+
+def faceless(mesh):
+    """The vertices of mesh with no faces."""
+    return sg.Mesh(mesh.pos, mesh.nrm, mesh.uv, np.zeros((0, 3), np.int32))

@@ -4,6 +4,7 @@ bar as the per-frame path) and the per-frame GPU path bit for bit."""
 import numpy as np
 import pytest
 
+from edge_scenes import empty_mesh
 from hana_amd import scenegen as sg
 from parity import assert_color_ok, assert_depth_exact
 from pyoracle import CPURenderer
@@ -127,3 +128,32 @@ def test_batch_rejects_shared_targets(ctx, scenes):
     with pytest.raises(HanaError):
         batch.finalize()
     batch.close()
+
+
+@pytest.mark.parametrize("empty_first", [False, True])
+def test_empty_mesh_frames(ctx, empty_first):
+    """A batch whose parts include one with only an empty mesh (no vertex, no
+    face), in a graph and eagerly, each run twice: that frame's target keeps
+    its contents and the other frame equals the oracle."""
+    from hana_amd.gpu import Batch, SceneOnDevice
+    o = CPURenderer("oracle")
+    full = sg.small_scene(sg.BLINN, shadow=True, seed=77, width=120, height=90)
+    empty = sg.Scene(empty_mesh(), 96, 64, sg.NORMALMAP, True, full.cam_pos, full.light_pos, full.diffuse,
+                     full.normal, name="empty")
+    scenes = [empty, full] if empty_first else [full, empty]
+    refs = [o.render_frame(s)[0] for s in scenes]
+    for graph in (True, False):
+        devs = [SceneOnDevice(ctx, s) for s in scenes]
+        batch = Batch(ctx)
+        for d in devs:
+            batch.add(d.desc())
+        batch.finalize(use_graph=graph)
+        for _ in range(2):
+            for d in devs:
+                d.clear()
+            batch.run()
+            for s, d, ref in zip(scenes, devs, refs):
+                col, dep = d.read()
+                assert_depth_exact(dep, ref.depth, s.name)
+                assert_color_ok(col, ref.color, s.name)
+        batch.close()

@@ -7,7 +7,7 @@ interpolated light-space depth) bit-exact.
 import numpy as np
 import pytest
 
-from edge_scenes import IDENTITY_VP, nan_depth_scene, sliver_scene, with_camera_vp
+from edge_scenes import IDENTITY_VP, empty_mesh, faceless, nan_depth_scene, sliver_scene, with_camera_vp
 from hana_amd import scenegen as sg
 from parity import assert_color_ok, assert_depth_exact
 from pyoracle import CPURenderer, HostTarget
@@ -201,6 +201,38 @@ def test_scattered_triangle_soup(ctx, oracle, n, wh, px):
         over += len(tiles) > slots
     assert over > n // 1024 // 2
     check_scene(ctx, oracle, scene)
+
+
+def test_empty_meshes(ctx, oracle):
+    """Draws of a mesh with no vertices, and of one with vertices but no
+    faces, right after draws that left tile counts and bins in both draw
+    slots: the target keeps its contents, or takes a pending clear."""
+    from hana_amd.gpu import build_uniforms
+    W, H = 150, 110
+    scenes = [sg.small_scene(k, seed=400 + i, width=W, height=H) for i, k in enumerate((sg.BLINN, sg.TEXTURE))]
+    tgt = ctx.target(W, H)
+    ref = HostTarget(W, H)
+    keep = []
+
+    def draw(mesh, sc):
+        u = build_uniforms(sc)
+        keep.append(u)
+        ctx.draw(ctx.mesh(mesh), sc.shader, u, tgt, ctx.texture(sc.diffuse), ctx.texture(sc.normal))
+        oracle.draw(mesh, sc.shader, oracle.build_uniforms(sc), ref, sc.diffuse, sc.normal)
+
+    for sc in scenes:
+        draw(sc.mesh, sc)
+    draw(empty_mesh(), scenes[0])
+    draw(empty_mesh(), scenes[1])
+    draw(faceless(scenes[0].mesh), scenes[0])
+    col, dep = ctx.read(tgt, W, H)
+    assert (dep < 1e30).sum() > 1000
+    assert_depth_exact(dep, ref.depth, "empty draws")
+    assert_color_ok(col, ref.color, "empty draws")
+    ctx.clear(tgt, (10, 20, 30, 255), 0.5)
+    draw(empty_mesh(), scenes[1])
+    col, dep = ctx.read(tgt, W, H)
+    assert (col == np.array([10, 20, 30, 255], np.uint8)).all() and (dep == np.float32(0.5)).all()
 
 
 def test_nan_depth_fragments(ctx, oracle):
