@@ -746,7 +746,8 @@ int hana_texture_destroy(hana_ctx* c, hana_texture* t) {
 }
 
 int hana_target_create(hana_ctx* c, int32_t w, int32_t h, hana_target** out) {
-    if (!c || !out || w <= 0 || h <= 0 || (int64_t)w * h > (1ll << 30))
+    // tile coordinates travel as 16-bit fields: at most 2^16 tiles a side
+    if (!c || !out || w <= 0 || h <= 0 || w > (1 << 20) || h > (1 << 20) || (int64_t)w * h > (1ll << 30))
         return fail(c, HANA_E_INVALID, "hana_target_create: bad size");
     hana_target* t = new hana_target();
     t->w = w;

@@ -150,7 +150,8 @@ int hana_texture_create(hana_ctx* ctx, int32_t w, int32_t h, int32_t bpp, const 
                         hana_texture** out);
 int hana_texture_destroy(hana_ctx* ctx, hana_texture* t);
 /* RenderBuffer(width, height): RGBA8 at (y*W+x)*4 and fp32 depth at y*W+x,
- * initialised like the reference constructor (colour 0,0,0,255, depth 1). */
+ * initialised like the reference constructor (colour 0,0,0,255, depth 1).
+ * 1 <= w, h <= 2^20 and w*h <= 2^30, else HANA_E_INVALID. */
 int hana_target_create(hana_ctx* ctx, int32_t w, int32_t h, hana_target** out);
 int hana_target_destroy(hana_ctx* ctx, hana_target* t);
 /* renderbuffer_clear_color + renderbuffer_clear_depth; deferred and fused

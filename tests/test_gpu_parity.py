@@ -235,6 +235,38 @@ def test_empty_meshes(ctx, oracle):
     assert (col == np.array([10, 20, 30, 255], np.uint8)).all() and (dep == np.float32(0.5)).all()
 
 
+def test_widest_target(ctx, oracle):
+    """A target 2^20 pixels wide (65,536 tiles in a row, the most the 16-bit
+    tile coordinates hold) renders like the oracle; one pixel wider is
+    refused."""
+    from hana_amd.gpu import HANA_E_INVALID, HanaError, build_uniforms
+    with pytest.raises(HanaError) as err:
+        ctx.target((1 << 20) + 1, 4)
+    assert err.value.code == HANA_E_INVALID
+    W, H = 1 << 20, 4
+    pos = np.array([[-0.99, -0.9, 0.2], [1.0, -0.95, 0.1], [1.0, 0.99, 0.3], [-0.98, 1.0, 0.4],
+                    [0.999, -1.0, -0.5], [1.0, -1.0, -0.5], [1.0, 1.0, -0.5]], np.float32)
+    idx = np.array([[0, 1, 2], [0, 2, 3], [4, 5, 6]], np.int32)
+    rng = np.random.default_rng(8)
+    mesh = sg.Mesh(pos, np.tile(np.array([[0.0, 0.0, 1.0]], np.float32), (7, 1)),
+                   rng.uniform(0, 1, (7, 2)).astype(np.float32), idx)
+    base = sg.small_scene(sg.TEXTURE, width=64, height=64, seed=8)
+    sc = sg.Scene(mesh, W, H, sg.TEXTURE, False, (0.0, 0.0, 2.0), base.light_pos, base.diffuse, base.normal,
+                  name="widest")
+    tgt = ctx.target(W, H, winner=True)
+    u = with_camera_vp(build_uniforms(sc), IDENTITY_VP)
+    ctx.draw(ctx.mesh(mesh), sc.shader, u, tgt, ctx.texture(sc.diffuse), ctx.texture(sc.normal))
+    col, dep = ctx.read(tgt, W, H)
+    win = ctx.read_winner(tgt, W, H)
+    ref = HostTarget(W, H)
+    _, owin = oracle.draw(mesh, sc.shader, with_camera_vp(oracle.build_uniforms(sc), IDENTITY_VP), ref, sc.diffuse,
+                          sc.normal, winner=True)
+    assert (owin[:, -16:] >= 0).any()                       # the last tile column has fragments
+    assert np.array_equal(win.astype(np.int64), as_u32_ids(owin))
+    assert_depth_exact(dep, ref.depth, "widest")
+    assert_color_ok(col, ref.color, "widest")
+
+
 def test_nan_depth_fragments(ctx, oracle):
     """Fragments of NaN depth (a vertex whose clip z and w overflow to inf):
     they replace any stored depth, every later fragment at their pixels
