@@ -730,6 +730,7 @@ __device__ __forceinline__ void bin_record(const PassDesc* passes, const Workspa
 template <int BITS>
 __global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* passes, Workspace ws, int nseg) {
     constexpr int BIN_HASH = 1 << BITS;
+    if (ws.status[STATUS_FLAGS] & FLAG_PAIR_LIMIT) return;   // nothing is rasterised
     if ((int)blockIdx.x < nseg) {
         __shared__ uint32_t keys[BIN_HASH];
         __shared__ uint32_t cnt[BIN_HASH];
@@ -799,13 +800,22 @@ __global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* passes, Workspace
 // SCAN_CHUNK counters; the apply kernel adds the sums of the chunks before.
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* sh) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+// Sums are 64-bit so that a total past 2^32 pairs is seen, not wrapped.
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+    uint32_t lo = __shfl_xor((uint32_t)v, m), hi = __shfl_xor((uint32_t)(v >> 32), m);
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t shfl_up64(uint64_t v, int d) {
+    uint32_t lo = __shfl_up((uint32_t)v, d), hi = __shfl_up((uint32_t)(v >> 32), d);
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t block_sum(uint64_t v, uint64_t* sh) {
+    for (int off = 32; off > 0; off >>= 1) v += shfl_xor64(v, off);
     int w = threadIdx.x >> 6;
     __syncthreads();
     if ((threadIdx.x & 63) == 0) sh[w] = v;
     __syncthreads();
-    uint32_t t = 0;
+    uint64_t t = 0;
     for (int i = 0; i < SCAN_THREADS / 64; i++) t += sh[i];
     return t;
 }
@@ -834,15 +844,15 @@ __device__ __forceinline__ void note_clip_needs(const Workspace& ws) {
 }
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(Workspace ws) {
-    __shared__ uint32_t sh[SCAN_THREADS / 64];
+    __shared__ uint64_t sh[SCAN_THREADS / 64];
     int n = ws.ntiles;
     int b = blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_PER_THREAD;
     uint32_t v[SCAN_PER_THREAD];
     load_counts(ws.tile_count, b, n, v);
-    uint32_t s = 0;
+    uint64_t s = 0;
 #pragma unroll
     for (int i = 0; i < SCAN_PER_THREAD; i++) s += v[i];
-    uint32_t t = block_sum(s, sh);
+    uint64_t t = block_sum(s, sh);
     if (threadIdx.x == 0) ws.scan_part[blockIdx.x] = t;
     if (blockIdx.x == 0 && threadIdx.x == 0) note_clip_needs(ws);
 }
@@ -850,59 +860,62 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(Workspace ws) {
 // With a single chunk k_scan_reduce is not launched and this kernel alone
 // scans the counts.
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(Workspace ws) {
-    __shared__ uint32_t sh[SCAN_THREADS / 64];
-    __shared__ uint32_t wsum[SCAN_THREADS / 64];
+    __shared__ uint64_t sh[SCAN_THREADS / 64];
+    __shared__ uint64_t wsum[SCAN_THREADS / 64];
     int n = ws.ntiles;
     int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (gridDim.x == 1 && tid == 0) note_clip_needs(ws);
     // sum of the chunks before this one
-    uint32_t before = 0;
+    uint64_t before = 0;
     for (int i = tid; i < (int)blockIdx.x; i += SCAN_THREADS) before += ws.scan_part[i];
     before = block_sum(before, sh);
     // this thread's counters, then an exclusive scan of the thread sums
     int b = blockIdx.x * SCAN_CHUNK + tid * SCAN_PER_THREAD;
     uint32_t v[SCAN_PER_THREAD];
     load_counts(ws.tile_count, b, n, v);
-    uint32_t s = 0;
+    uint64_t s = 0;
 #pragma unroll
     for (int i = 0; i < SCAN_PER_THREAD; i++) s += v[i];
-    uint32_t incl = s;
+    uint64_t incl = s;
     for (int d = 1; d < 64; d <<= 1) {
-        uint32_t o = __shfl_up(incl, d);
+        uint64_t o = shfl_up64(incl, d);
         if (lane >= d) incl += o;
     }
     if (lane == 63) wsum[w] = incl;
     __syncthreads();
-    uint32_t wbefore = 0;
+    uint64_t wbefore = 0;
     for (int i = 0; i < w; i++) wbefore += wsum[i];
-    uint32_t run = before + wbefore + incl - s;
+    uint64_t run = before + wbefore + incl - s;
+    // offsets past MAX_PAIRS are cut off (the sequence then rasterises nothing)
     if (b + SCAN_PER_THREAD <= n) {
         uint4* qo = reinterpret_cast<uint4*>(ws.tile_offset + b);
         uint4* qc = reinterpret_cast<uint4*>(ws.tile_cursor + b);
 #pragma unroll
         for (int i = 0; i < SCAN_PER_THREAD / 4; i++) {
             uint4 x;
-            x.x = run; run += v[4 * i];
-            x.y = run; run += v[4 * i + 1];
-            x.z = run; run += v[4 * i + 2];
-            x.w = run; run += v[4 * i + 3];
+            x.x = (uint32_t)min(run, (uint64_t)MAX_PAIRS); run += v[4 * i];
+            x.y = (uint32_t)min(run, (uint64_t)MAX_PAIRS); run += v[4 * i + 1];
+            x.z = (uint32_t)min(run, (uint64_t)MAX_PAIRS); run += v[4 * i + 2];
+            x.w = (uint32_t)min(run, (uint64_t)MAX_PAIRS); run += v[4 * i + 3];
             qo[i] = x;
             qc[i] = x;
         }
     } else {
         for (int i = 0; i < SCAN_PER_THREAD; i++)
             if (b + i < n) {
-                ws.tile_offset[b + i] = run;
-                ws.tile_cursor[b + i] = run;
+                ws.tile_offset[b + i] = (uint32_t)min(run, (uint64_t)MAX_PAIRS);
+                ws.tile_cursor[b + i] = (uint32_t)min(run, (uint64_t)MAX_PAIRS);
                 run += v[i];
             }
     }
     if (blockIdx.x == gridDim.x - 1 && tid == SCAN_THREADS - 1) {
         // `run` of the last thread is the grand total
-        ws.tile_offset[n] = run;
-        ws.status[STATUS_BINS_NEEDED] = run;
-        atomicMax(&ws.status[STATUS_NEED_BINS], run);
+        uint32_t total = (uint32_t)min(run, (uint64_t)MAX_PAIRS);
+        ws.tile_offset[n] = total;
+        ws.status[STATUS_BINS_NEEDED] = total;
+        atomicMax(&ws.status[STATUS_NEED_BINS], total);
         if (run > ws.bin_cap) atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_BIN_OVERFLOW);
+        if (run > MAX_PAIRS) atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_PAIR_LIMIT);
         // every status word is final now (binning only reads them)
         if (ws.status_out)
             for (int i = 0; i < STATUS_WORDS; i++)
@@ -1138,7 +1151,8 @@ __global__ __launch_bounds__(RBLOCK) void k_raster(const PassDesc* passes, int n
 
     uint32_t beg = ws.tile_offset[gt];
     uint32_t cnt = ws.tile_count[gt];
-    if (beg >= ws.bin_cap) cnt = 0;
+    // past MAX_PAIRS the bins are not written and their positions may wrap
+    if (beg >= ws.bin_cap || (ws.status[STATUS_FLAGS] & FLAG_PAIR_LIMIT)) cnt = 0;
     else if (cnt > ws.bin_cap - beg) cnt = ws.bin_cap - beg;
     sweep(L, P, ws, beg, cnt, xb, yb, false);
 

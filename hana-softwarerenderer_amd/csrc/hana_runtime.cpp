@@ -182,11 +182,11 @@ int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_p
     if ((r = ensure(c, A.tile_count, sizeof(uint32_t) * nt))) return r;
     if ((r = ensure(c, A.tile_offset, sizeof(uint32_t) * nt))) return r;
     if ((r = ensure(c, A.tile_cursor, sizeof(uint32_t) * nt))) return r;
-    if ((r = ensure(c, A.scan_part, sizeof(uint32_t) * (size_t)scan_chunks(pl.total_tiles)))) return r;
+    if ((r = ensure(c, A.scan_part, sizeof(uint64_t) * (size_t)scan_chunks(pl.total_tiles)))) return r;
     // (triangle, tile) entries: most triangles touch one tile and about half
     // are culled; the scan reports the exact need if this is exceeded
     uint64_t want_bins = std::max<uint64_t>({(uint64_t)min_bins, (uint64_t)4 * pl.total_f + 2 * nt, 1u << 22});
-    want_bins = std::min<uint64_t>(want_bins, 0xFFFFFFF0ull);
+    want_bins = std::min<uint64_t>(want_bins, MAX_PAIRS);
     if (want_bins > A.bin_cap) {
         if ((r = ensure(c, A.bins, sizeof(uint32_t) * want_bins))) return r;
         A.bin_cap = (uint32_t)want_bins;
@@ -257,7 +257,7 @@ int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_p
     ws.seg_wide = A.seg_wide.as<uint16_t>();
     ws.seg_info = A.seg_info.as<uint32_t>();
     ws.seg_stride = seg_stride;
-    ws.scan_part = A.scan_part.as<uint32_t>();
+    ws.scan_part = A.scan_part.as<uint64_t>();
     ws.clipq_cap = A.clipq_cap;
     ws.status = A.status.as<uint32_t>();
     ws.status_out = nullptr;
@@ -397,7 +397,7 @@ static int grow_arena(hana_ctx* c, Arena& A, uint32_t flags, uint32_t need_bins,
                       uint32_t need_tris) {
     if (flags & FLAG_BIN_OVERFLOW) {
         uint64_t want = std::max<uint64_t>((uint64_t)need_bins + need_bins / 4 + 1024, A.bin_cap);
-        want = std::min<uint64_t>(want, 0xFFFFFFF0ull);
+        want = std::min<uint64_t>(want, MAX_PAIRS);
         int r = ensure(c, A.bins, sizeof(uint32_t) * want);
         if (r) return r;
         A.bin_cap = (uint32_t)want;
@@ -453,6 +453,10 @@ int check_status(hana_ctx* c) {
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     if (flags & FLAG_POLY_TOO_LARGE)
         return fail(c, HANA_E_OVERFLOW, "a clipped polygon exceeded 10 vertices and was dropped");
+    if (flags & FLAG_PAIR_LIMIT)
+        return fail(c, HANA_E_NOMEM,
+                    "a draw, frame or batch part covered more than 4,294,967,280 (triangle, tile) pairs and was not "
+                    "rasterised; split it");
     // grow for the next call; the frames since the last check must be re-rendered
     for (int k = 0; k < 2; k++)
         if ((r = grow_arena(c, *arena[k], flags, need_bins, need_polys, need_tris))) return r;

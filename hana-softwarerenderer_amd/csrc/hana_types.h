@@ -123,7 +123,7 @@ struct Workspace {
     uint16_t* seg_wide;          // per segment, BLOCK*FPT entries: its triangles over more tiles
     uint32_t* seg_info;          // per segment: listed tiles | wide triangles << 16
     uint32_t seg_stride;
-    uint32_t* scan_part;         // per scan chunk: sum of its tile counts
+    uint64_t* scan_part;         // per scan chunk: sum of its tile counts
     uint32_t* status;            // see STATUS_*
     uint32_t* status_out;        // or null: host-mapped copy of the status words, written by the scan
     int32_t ntiles;
@@ -148,7 +148,10 @@ constexpr int SCAN_CHUNK = SCAN_THREADS * SCAN_PER_THREAD;
 enum { STATUS_POLYS = 0, STATUS_CLIPQ = 1, STATUS_TRIS = 2, STATUS_BINS_NEEDED = 3, STATUS_SEQ_WORDS = 4,
        STATUS_FLAGS = 4, STATUS_NEED_BINS = 5, STATUS_NEED_POLYS = 6, STATUS_NEED_TRIS = 7, STATUS_WORDS = 8 };
 enum { FLAG_BIN_OVERFLOW = 1, FLAG_POLY_OVERFLOW = 2, FLAG_POLY_TOO_LARGE = 4, FLAG_CLIPQ_OVERFLOW = 8,
-       FLAG_TRIS_OVERFLOW = 16 };
+       FLAG_TRIS_OVERFLOW = 16, FLAG_PAIR_LIMIT = 32 };
+// (triangle, tile) pairs one launch sequence may hold: bin positions are
+// 32-bit.  A sequence with more sets FLAG_PAIR_LIMIT and rasterises nothing.
+constexpr uint32_t MAX_PAIRS = 0xFFFFFFF0u;
 // vertex outcode bits
 constexpr uint32_t CODE_VISIBLE = 0x80u;
 constexpr uint32_t CODE_PLANES = 0x7Fu;

@@ -392,6 +392,34 @@ def test_bin_overflow_regrows(oracle):
         assert_color_ok(col, ref.color, "slivers")
 
 
+def test_pair_limit_refused(oracle):
+    """A draw whose triangles cover more (triangle, tile) pairs than 32-bit
+    bin positions reach (32,768 bands of 196,608 tiles each on a 2^20 x 2^10
+    target: 6.4 G pairs) is refused with HANA_E_NOMEM at the next sync and
+    bins nothing; the context keeps rendering.  A context of its own, which
+    frees the 8 GB target at the end."""
+    from hana_amd.gpu import HANA_E_NOMEM, Context, HanaError, build_uniforms
+    W, H, n = 1 << 20, 1 << 10, 32768
+    rng = np.random.default_rng(12)
+    ty = rng.integers(0, H // 16 - 3, n)
+    y0, y1 = (ty * 16 + 8) / H * 2 - 1, (ty * 16 + 40) / H * 2 - 1    # 3 tile rows, the full width
+    z = rng.uniform(-0.5, 0.5, n)
+    tri = np.stack([np.stack([-np.ones(n), y0, z], 1), np.stack([np.ones(n), y0, z], 1),
+                    np.stack([np.ones(n), y1, z], 1)], 1)
+    mesh = sg.Mesh(tri.reshape(-1, 3).astype(np.float32), np.tile(np.array([[0.0, 0.0, 1.0]], np.float32), (3 * n, 1)),
+                   rng.uniform(0, 1, (3 * n, 2)).astype(np.float32), np.arange(3 * n, dtype=np.int32).reshape(n, 3))
+    base = sg.small_scene(sg.TEXTURE, width=64, height=64, seed=12)
+    bands = sg.Scene(mesh, W, H, sg.TEXTURE, False, (0.0, 0.0, 2.0), base.light_pos, base.diffuse, base.normal)
+    with Context(0) as ctx:
+        big = ctx.target(W, H)
+        u = with_camera_vp(build_uniforms(bands), IDENTITY_VP)
+        ctx.draw(ctx.mesh(mesh), bands.shader, u, big, ctx.texture(bands.diffuse), ctx.texture(bands.normal))
+        with pytest.raises(HanaError) as err:
+            ctx.sync()
+        assert err.value.code == HANA_E_NOMEM and "(triangle, tile) pairs" in str(err.value)
+        check_scene(ctx, oracle, sg.small_scene(sg.BLINN, shadow=True, seed=6, width=96, height=72))
+
+
 def test_clip_pool_overflow_in_a_batch(ctx, oracle):
     """Batch finalisation sizes the pool and the lists with its trial run."""
     from hana_amd.gpu import Batch, SceneOnDevice
