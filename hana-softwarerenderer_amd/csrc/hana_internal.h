@@ -78,6 +78,14 @@ struct Plan {
 };
 
 constexpr int kRing = 32;
+// streams (each with a workspace) that consecutive per-call draws rotate over
+#ifndef HANA_DRAW_SLOTS
+#define HANA_DRAW_SLOTS 3
+#endif
+constexpr int kDrawSlots = HANA_DRAW_SLOTS;
+// launch sequences of at least this many faces use two of the slots
+constexpr size_t kLargeSequenceFaces = size_t(1) << 19;
+static_assert(kDrawSlots >= 2, "batches run their second part on the stream of draw slot 1");
 
 }  // namespace rt
 }  // namespace hana
@@ -100,16 +108,17 @@ struct hana_ctx {
     uint64_t* stats_host = nullptr;    // pinned, per pass
     int stats_passes = 0;
     bool status_pending = false;
-    // pipelined draws: consecutive draws alternate between two slots, each a
-    // stream with its own workspace ((stream, arena), (stream2, arena2)), so
-    // the geometry stages of one draw overlap the raster of the one before;
+    // pipelined draws: consecutive draws rotate over kDrawSlots slots, each a
+    // stream with its own workspace and status words (slot 0 is (stream,
+    // arena, status_host), slot k > 0 is (xstream, xarena, xstatus_host)[k-1]),
+    // so the geometry stages of a draw overlap the rasters of the draws before;
     // each raster waits for the whole previous draw
-    hipStream_t stream2 = nullptr;
-    Arena arena2;
-    uint32_t* status_host2 = nullptr;
-    uint32_t* status_dev[2] = {};      // device addresses of status_host, status_host2
-    hipEvent_t slot_done[2] = {};
-    bool slot_pending[2] = {};         // a draw of slot k the other stream has not waited for
+    hipStream_t xstream[hana::rt::kDrawSlots - 1] = {};
+    Arena xarena[hana::rt::kDrawSlots - 1];
+    uint32_t* xstatus_host[hana::rt::kDrawSlots - 1] = {};
+    uint32_t* status_dev[hana::rt::kDrawSlots] = {};   // device addresses of the slots' status words
+    hipEvent_t slot_done[hana::rt::kDrawSlots] = {};
+    bool slot_pending[hana::rt::kDrawSlots] = {};      // a draw of slot k no other stream has waited for
     int next_slot = 0;
     bool pipelined = true;
     // device staging for hana_target_present
@@ -121,9 +130,9 @@ struct hana_ctx {
 };
 
 // A batch's frames are split into up to kBatchParts parts, each a launch
-// sequence with its own workspace; the parts run concurrently, part 0 on the
-// context stream, part 1 on its second stream and further parts on streams
-// of the batch (branches of one graph), since their frames share no target.
+// sequence with its own workspace; the parts run concurrently, part p on the
+// stream of draw slot p and parts beyond the slots on streams of the batch
+// (branches of one graph), since their frames share no target.
 struct hana_batch_part {
     std::vector<hana::rt::PassSpec> specs;
     hana::rt::Plan plan;
@@ -140,7 +149,7 @@ struct hana_batch {
     hipGraphExec_t exec = nullptr;
     hipEvent_t fork = nullptr;
     std::vector<hipEvent_t> joined;    // one per part after the first
-    std::vector<hipStream_t> extra;    // streams of parts 2, 3, ...
+    std::vector<hipStream_t> extra;    // streams of the parts beyond the context's draw slots
 };
 
 namespace hana {
@@ -150,10 +159,14 @@ int fail(hana_ctx* c, int code, const std::string& msg);
 int hip_fail(hana_ctx* c, hipError_t e, const char* what);
 int ensure(hana_ctx* c, DevBuf& b, size_t bytes);
 int materialize_clear(hana_ctx* c, hana_target* t, hipStream_t s);
-// Orders the context stream after every draw in flight on the second stream.
+// Orders the context stream after every draw in flight on another slot's stream.
 int join(hana_ctx* c);
-// Waits for both streams of the context.
+// Waits for every stream of the context.
 int sync_all(hana_ctx* c);
+// Draw slot k's stream, workspace and host status words (slot 0: the context's own).
+hipStream_t slot_stream(hana_ctx* c, int k);
+Arena& slot_arena(hana_ctx* c, int k);
+uint32_t* slot_status(hana_ctx* c, int k);
 int check_status(hana_ctx* c);
 // Enqueues the launch sequence of a set of passes on the context's stream.
 int run_passes(hana_ctx* c, std::vector<PassSpec> specs);

@@ -41,17 +41,22 @@ int hip_fail(hana_ctx* c, hipError_t e, const char* what) {
         if (e_ != hipSuccess) return hip_fail(c, e_, #expr); \
     } while (0)
 
+hipStream_t slot_stream(hana_ctx* c, int k) { return k ? c->xstream[k - 1] : c->stream; }
+Arena& slot_arena(hana_ctx* c, int k) { return k ? c->xarena[k - 1] : c->arena; }
+uint32_t* slot_status(hana_ctx* c, int k) { return k ? c->xstatus_host[k - 1] : c->status_host; }
+
 int sync_all(hana_ctx* c) {
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (c->stream2) HIP_TRY(c, hipStreamSynchronize(c->stream2));
+    for (int k = 0; k < kDrawSlots; k++)
+        if (slot_stream(c, k)) HIP_TRY(c, hipStreamSynchronize(slot_stream(c, k)));
     return HANA_OK;
 }
 
 int join(hana_ctx* c) {
-    if (c->slot_pending[1]) {
-        HIP_TRY(c, hipStreamWaitEvent(c->stream, c->slot_done[1], 0));
-        c->slot_pending[1] = false;
-    }
+    for (int k = 1; k < kDrawSlots; k++)
+        if (c->slot_pending[k]) {
+            HIP_TRY(c, hipStreamWaitEvent(c->stream, c->slot_done[k], 0));
+            c->slot_pending[k] = false;
+        }
     return HANA_OK;
 }
 
@@ -422,8 +427,8 @@ static int grow_arena(hana_ctx* c, Arena& A, uint32_t flags, uint32_t need_bins,
     return HANA_OK;
 }
 
-// Reads the sticky status of every sequence since the last check, over both
-// draw slots (waits for both streams).  On an overflow both workspaces are
+// Reads the sticky status of every sequence since the last check, over all
+// draw slots (waits for every stream).  On an overflow both workspaces are
 // grown to the largest need seen, the sticky words are cleared, and
 // HANA_E_OVERFLOW asks the caller to render again.
 int check_status(hana_ctx* c) {
@@ -431,23 +436,24 @@ int check_status(hana_ctx* c) {
     int r = sync_all(c);
     if (r) return r;
     c->status_pending = false;
-    uint32_t* host[2] = {c->status_host, c->status_host2};
-    Arena* arena[2] = {&c->arena, &c->arena2};
     uint32_t flags = 0, need_bins = 0, need_polys = 0, need_tris = 0;
-    for (int k = 0; k < 2; k++) {
-        if (!host[k]) continue;
-        flags |= host[k][STATUS_FLAGS];
-        need_bins = std::max(need_bins, host[k][STATUS_NEED_BINS]);
-        need_polys = std::max(need_polys, host[k][STATUS_NEED_POLYS]);
-        need_tris = std::max(need_tris, host[k][STATUS_NEED_TRIS]);
+    for (int k = 0; k < kDrawSlots; k++) {
+        const uint32_t* host = slot_status(c, k);
+        if (!host) continue;
+        flags |= host[STATUS_FLAGS];
+        need_bins = std::max(need_bins, host[STATUS_NEED_BINS]);
+        need_polys = std::max(need_polys, host[STATUS_NEED_POLYS]);
+        need_tris = std::max(need_tris, host[STATUS_NEED_TRIS]);
     }
     if (!flags) return HANA_OK;
-    for (int k = 0; k < 2; k++) {
-        if (!host[k]) continue;
-        if (arena[k]->status.p)
-            HIP_TRY(c, launch_fill32(arena[k]->status.as<uint32_t>() + STATUS_SEQ_WORDS, 0u,
-                                     STATUS_WORDS - STATUS_SEQ_WORDS, c->stream));
-        std::memset(host[k] + STATUS_SEQ_WORDS, 0, sizeof(uint32_t) * (STATUS_WORDS - STATUS_SEQ_WORDS));
+    for (int k = 0; k < kDrawSlots; k++) {
+        uint32_t* host = slot_status(c, k);
+        if (!host) continue;
+        Arena& A = slot_arena(c, k);
+        if (A.status.p)
+            HIP_TRY(c, launch_fill32(A.status.as<uint32_t>() + STATUS_SEQ_WORDS, 0u, STATUS_WORDS - STATUS_SEQ_WORDS,
+                                     c->stream));
+        std::memset(host + STATUS_SEQ_WORDS, 0, sizeof(uint32_t) * (STATUS_WORDS - STATUS_SEQ_WORDS));
     }
     // the next geometry stages may run on either stream
     HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -458,17 +464,17 @@ int check_status(hana_ctx* c) {
                     "a draw, frame or batch part covered more than 4,294,967,280 (triangle, tile) pairs and was not "
                     "rasterised; split it");
     // grow for the next call; the frames since the last check must be re-rendered
-    for (int k = 0; k < 2; k++)
-        if ((r = grow_arena(c, *arena[k], flags, need_bins, need_polys, need_tris))) return r;
+    for (int k = 0; k < kDrawSlots; k++)
+        if ((r = grow_arena(c, slot_arena(c, k), flags, need_bins, need_polys, need_tris))) return r;
     return fail(c, HANA_E_OVERFLOW,
                 "bin or clip-pool capacity exceeded since the last sync; buffers were grown, re-render the frame");
 }
 
-// Renders a list of passes (per-call path).  Consecutive calls alternate
-// between two slots, each a stream with its own workspace: the geometry
-// stages of a call run while the previous call rasterises, and its raster
-// waits for the previous call to finish, so targets and shadow maps see the
-// calls in order.  With timing or statistics on, every call uses slot 0.
+// Renders a list of passes (per-call path).  Consecutive calls rotate over
+// kDrawSlots slots, each a stream with its own workspace: the geometry
+// stages of a call run while earlier calls rasterise, and its raster waits
+// for the previous call to finish, so targets and shadow maps see the calls
+// in order.  With timing or statistics on, every call uses slot 0.
 int run_passes(hana_ctx* c, std::vector<PassSpec> specs) {
     int r;
     std::vector<hana_target*> sampled;   // sampled as a shadow map, not drawn here
@@ -480,9 +486,19 @@ int run_passes(hana_ctx* c, std::vector<PassSpec> specs) {
     }
     Plan pl;
     if ((r = build_plan(c, std::move(specs), pl))) return r;
-    int a = (c->pipelined && c->stream2 && !c->timing && !c->stats_enabled) ? c->next_slot : 0;
-    hipStream_t S = a ? c->stream2 : c->stream;
-    Arena& A = a ? c->arena2 : c->arena;
+    bool rotate = c->pipelined && !c->timing && !c->stats_enabled;
+    // Large sequences rotate over two slots: with a third in flight, the 2.6 M
+    // faces of a 1.3 M-triangle frame with its shadow pass ran 1.2 % slower,
+    // while frames of up to 400 k faces ran 6 % faster.
+    int nslots = pl.total_f >= kLargeSequenceFaces ? 2 : kDrawSlots;
+    int a = rotate ? c->next_slot % nslots : 0;
+    hipStream_t S = slot_stream(c, a);
+    Arena& A = slot_arena(c, a);
+    // the slots' workspaces grow together, so that a sequence of calls
+    // allocates (and waits for the streams) once, not once per slot
+    if (rotate)
+        for (int k = 0; k < nslots; k++)
+            if (k != a && (r = bind_plan(c, pl, slot_arena(c, k), 0, 0, 0, false, slot_stream(c, k)))) return r;
     if ((r = bind_plan(c, pl, A, 0, 0, 0, c->stats_enabled, S))) return r;
     pl.ws.status_out = c->status_dev[a];   // the scan leaves the status words in host memory
     // a target sampled as a shadow map by a pass of the same plan must be drawn
@@ -494,10 +510,11 @@ int run_passes(hana_ctx* c, std::vector<PassSpec> specs) {
         HIP_TRY(c, launch_fill32(reinterpret_cast<uint32_t*>(A.stats.p), 0u, (size_t)ST_WORDS * 2 * n, S));
     }
     if ((r = enqueue_geometry(c, pl, A, S, true))) return r;
-    if (c->slot_pending[1 - a]) {
-        HIP_TRY(c, hipStreamWaitEvent(S, c->slot_done[1 - a], 0));
-        c->slot_pending[1 - a] = false;
-    }
+    for (int k = 0; k < kDrawSlots; k++)
+        if (k != a && c->slot_pending[k]) {
+            HIP_TRY(c, hipStreamWaitEvent(S, c->slot_done[k], 0));
+            c->slot_pending[k] = false;
+        }
     for (hana_target* t : sampled)
         if ((r = materialize_clear(c, t, S))) return r;
     if ((r = enqueue_raster(c, pl, A, S, true))) return r;
@@ -514,7 +531,7 @@ int run_passes(hana_ctx* c, std::vector<PassSpec> specs) {
         c->stats_passes = n;
     }
     c->slot_pending[a] = true;
-    c->next_slot = 1 - a;
+    c->next_slot = (a + 1) % nslots;
     c->status_pending = true;
     return HANA_OK;
 }
@@ -541,12 +558,11 @@ int hana_gpu_create(int device, hana_ctx** out) {
     auto undo = [c]() {
         for (int i = 0; i < kRing; i++)
             if (c->ring_event[i]) (void)hipEventDestroy(c->ring_event[i]);
-        for (int i = 0; i < 2; i++)
-            if (c->slot_done[i]) (void)hipEventDestroy(c->slot_done[i]);
-        if (c->status_host) (void)hipHostFree(c->status_host);
-        if (c->status_host2) (void)hipHostFree(c->status_host2);
-        if (c->stream) (void)hipStreamDestroy(c->stream);
-        if (c->stream2) (void)hipStreamDestroy(c->stream2);
+        for (int k = 0; k < kDrawSlots; k++) {
+            if (c->slot_done[k]) (void)hipEventDestroy(c->slot_done[k]);
+            if (slot_status(c, k)) (void)hipHostFree(slot_status(c, k));
+            if (slot_stream(c, k)) (void)hipStreamDestroy(slot_stream(c, k));
+        }
         (void)hipGetLastError();
         delete c;
         return HANA_E_DEVICE;
@@ -555,16 +571,14 @@ int hana_gpu_create(int device, hana_ctx** out) {
         return undo();
     for (int i = 0; i < kRing; i++)
         if (hipEventCreateWithFlags(&c->ring_event[i], hipEventDisableTiming) != hipSuccess) return undo();
-    if (hipHostMalloc((void**)&c->status_host, sizeof(uint32_t) * STATUS_WORDS) != hipSuccess) return undo();
-    std::memset(c->status_host, 0, sizeof(uint32_t) * STATUS_WORDS);
-    if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) return undo();
-    for (int i = 0; i < 2; i++)
-        if (hipEventCreateWithFlags(&c->slot_done[i], hipEventDisableTiming) != hipSuccess) return undo();
-    if (hipHostMalloc((void**)&c->status_host2, sizeof(uint32_t) * STATUS_WORDS) != hipSuccess) return undo();
-    std::memset(c->status_host2, 0, sizeof(uint32_t) * STATUS_WORDS);
-    if (hipHostGetDevicePointer((void**)&c->status_dev[0], c->status_host, 0) != hipSuccess ||
-        hipHostGetDevicePointer((void**)&c->status_dev[1], c->status_host2, 0) != hipSuccess)
-        return undo();
+    for (int k = 0; k < kDrawSlots; k++) {
+        if (k && hipStreamCreateWithFlags(&c->xstream[k - 1], hipStreamNonBlocking) != hipSuccess) return undo();
+        uint32_t*& host = k ? c->xstatus_host[k - 1] : c->status_host;
+        if (hipHostMalloc((void**)&host, sizeof(uint32_t) * STATUS_WORDS) != hipSuccess) return undo();
+        std::memset(host, 0, sizeof(uint32_t) * STATUS_WORDS);
+        if (hipHostGetDevicePointer((void**)&c->status_dev[k], host, 0) != hipSuccess) return undo();
+        if (hipEventCreateWithFlags(&c->slot_done[k], hipEventDisableTiming) != hipSuccess) return undo();
+    }
     *out = c;
     return HANA_OK;
 }
@@ -584,27 +598,26 @@ static void free_arena(Arena& a) {
 int hana_gpu_destroy(hana_ctx* c) {
     if (!c) return HANA_E_INVALID;
     (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
-    (void)hipStreamSynchronize(c->stream2);
-    free_arena(c->arena);
-    free_arena(c->arena2);
+    for (int k = 0; k < kDrawSlots; k++) {
+        (void)hipStreamSynchronize(slot_stream(c, k));
+        free_arena(slot_arena(c, k));
+    }
     if (c->present.p) (void)hipFree(c->present.p);
     for (int i = 0; i < kRing; i++) {
         if (c->ring_host[i]) (void)hipHostFree(c->ring_host[i]);
         if (c->ring_event[i]) (void)hipEventDestroy(c->ring_event[i]);
     }
-    if (c->status_host) (void)hipHostFree(c->status_host);
-    if (c->status_host2) (void)hipHostFree(c->status_host2);
     if (c->stats_host) (void)hipHostFree(c->stats_host);
-    for (int i = 0; i < 2; i++)
-        if (c->slot_done[i]) (void)hipEventDestroy(c->slot_done[i]);
+    for (int k = 0; k < kDrawSlots; k++) {
+        if (slot_status(c, k)) (void)hipHostFree(slot_status(c, k));
+        if (c->slot_done[k]) (void)hipEventDestroy(c->slot_done[k]);
+    }
     for (auto& p : c->ev_pending) {
         (void)hipEventDestroy(p.second.first);
         (void)hipEventDestroy(p.second.second);
     }
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
-    (void)hipStreamDestroy(c->stream);
-    (void)hipStreamDestroy(c->stream2);
+    for (int k = 0; k < kDrawSlots; k++) (void)hipStreamDestroy(slot_stream(c, k));
     delete c;
     return HANA_OK;
 }
@@ -965,10 +978,16 @@ static void batch_split(hana_batch* b) {
     }
 }
 
-// Enqueues every part: part 0 on the context stream, the others on the
-// second stream and the batch's own streams, forked from and joined back
-// into the context stream (with stage timing on, all run on the context
-// stream so that stage times do not overlap).
+// Part p of a batch runs on the stream of draw slot p (part 0 on the
+// context stream), and parts beyond the slots on the batch's own streams, so
+// that a context and its batches use as few streams as possible.
+static hipStream_t part_stream(hana_ctx* c, hana_batch* b, size_t p) {
+    return p < (size_t)kDrawSlots ? slot_stream(c, (int)p) : b->extra[p - kDrawSlots];
+}
+
+// Enqueues every part on its stream, forked from and joined back into the
+// context stream (with stage timing on, all run on the context stream so
+// that stage times do not overlap).
 static int batch_enqueue(hana_ctx* c, hana_batch* b, bool timed) {
     int r;
     if (b->parts.size() == 1 || (timed && c->timing)) {
@@ -978,15 +997,14 @@ static int batch_enqueue(hana_ctx* c, hana_batch* b, bool timed) {
     }
     HIP_TRY(c, hipEventRecord(b->fork, c->stream));
     for (size_t p = 1; p < b->parts.size(); p++) {
-        hipStream_t s = p == 1 ? c->stream2 : b->extra[p - 2];
-        HIP_TRY(c, hipStreamWaitEvent(s, b->fork, 0));
+        HIP_TRY(c, hipStreamWaitEvent(part_stream(c, b, p), b->fork, 0));
     }
     for (size_t p = 0; p < b->parts.size(); p++) {
-        hipStream_t s = p == 0 ? c->stream : p == 1 ? c->stream2 : b->extra[p - 2];
+        hipStream_t s = part_stream(c, b, p);
         if ((r = enqueue_sequence(c, b->parts[p].plan, b->parts[p].arena, s, timed))) return r;
     }
     for (size_t p = 1; p < b->parts.size(); p++) {
-        hipStream_t s = p == 1 ? c->stream2 : b->extra[p - 2];
+        hipStream_t s = part_stream(c, b, p);
         HIP_TRY(c, hipEventRecord(b->joined[p - 1], s));
         HIP_TRY(c, hipStreamWaitEvent(c->stream, b->joined[p - 1], 0));
     }
@@ -1042,7 +1060,7 @@ int hana_batch_finalize(hana_ctx* c, hana_batch* b, int use_graph) {
         HIP_TRY(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
         b->joined.push_back(e);
     }
-    while (b->extra.size() + 2 < b->parts.size()) {
+    while (b->extra.size() + kDrawSlots < b->parts.size()) {
         hipStream_t s = nullptr;
         HIP_TRY(c, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         b->extra.push_back(s);

@@ -22,12 +22,13 @@
  *   hana_scene_*         Scene / SingleModelScene state, orbit      SR/scene.cpp:3-9,45-63,127-147,
  *                        camera and its reset                       SR/camera.cpp:31-70,167-169
  *
- * A context owns two HIP streams, each with its own workspace, and every
+ * A context owns three HIP streams, each with its own workspace, and every
  * device object created on it; calls on one context must be externally
- * synchronised.  Work is enqueued asynchronously: consecutive draws alternate
- * between the two streams so that one draw's geometry stages overlap the
- * previous draw's raster, and every raster waits for the previous draw, so
- * targets see the draws in call order.  hana_target_read(_winner),
+ * synchronised.  Work is enqueued asynchronously: consecutive draws rotate
+ * over the streams (over two of them for draws of 2^19 faces or more) so that
+ * a draw's geometry stages overlap the rasters of the draws before it, and
+ * every raster waits for the previous draw, so targets see the draws in call
+ * order.  hana_target_read(_winner),
  * hana_target_present and hana_gpu_sync wait for all of it.  If a draw since
  * the previous such wait outgrew the workspace, the wait returns
  * HANA_E_OVERFLOW after growing it: render those frames again.

@@ -1,10 +1,11 @@
 """Consecutive draws with no read in between, on small (150x110) and large
-(2048x2048) targets.  The per-call path alternates two streams with their
-own workspaces, so a draw's geometry stages overlap the previous draw's
-raster; the results must still be those of the calls run one after another:
+(2048x2048) targets.  The per-call path rotates over streams with their
+own workspaces, so a draw's geometry stages overlap the rasters before it;
+the results must still be those of the calls run one after another:
 depth tests across draws, shadow maps written by one call and sampled by the
-next, targets read, uploaded or presented between calls, and timing switched
-on in the middle of a chain."""
+next, targets read, uploaded or presented between calls, timing switched
+on in the middle of a chain, and chains that mix calls of at least 2^19 faces
+(which rotate over two slots) with smaller ones (three slots)."""
 import numpy as np
 import pytest
 
@@ -122,3 +123,25 @@ def test_reads_uploads_and_presents_between_draws(ctx, oracle, wh):
     col, dep = ctx.read(tgt, W, H)
     assert_depth_exact(dep, ref.depth, "after upload")
     assert_color_ok(col, ref.color, "after upload")
+
+
+def test_large_and_small_calls_mixed(ctx, oracle, wh):
+    """Calls of 588 k faces (two slots) between calls of a few thousand
+    (three slots), into one target: every raster waits for the call before
+    it whatever slots the two use."""
+    from hana_amd.gpu import build_uniforms
+    W, H = wh
+    big = sg.small_scene(sg.BLINN, seed=330, width=W, height=H, segments=768, rings=384, cam=(0.3, 0.2, 1.9))
+    small = chain_scenes(4, 340, wh)
+    order = [small[0], big, small[1], small[2], big, small[3], big]
+    tgt = ctx.target(W, H)
+    keep, meshes = [], {}
+    for sc in order:
+        u = build_uniforms(sc)
+        keep.append(u)
+        m = meshes.setdefault(id(sc), ctx.mesh(sc.mesh))
+        ctx.draw(m, sc.shader, u, tgt, ctx.texture(sc.diffuse), ctx.texture(sc.normal))
+    col, dep = ctx.read(tgt, W, H)
+    ref = oracle_chain(oracle, order)
+    assert_depth_exact(dep, ref.depth, "mixed")
+    assert_color_ok(col, ref.color, "mixed")
