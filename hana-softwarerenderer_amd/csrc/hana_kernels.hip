@@ -733,11 +733,8 @@ __global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* passes, Workspace
     if (ws.status[STATUS_FLAGS] & FLAG_PAIR_LIMIT) return;   // nothing is rasterised
     if ((int)blockIdx.x < nseg) {
         __shared__ uint32_t keys[BIN_HASH];
-        __shared__ uint32_t cnt[BIN_HASH];
-        for (int i = threadIdx.x; i < BIN_HASH; i += BLOCK) {
-            keys[i] = BIN_EMPTY;
-            cnt[i] = 0u;
-        }
+        __shared__ uint32_t cnt[BIN_HASH];   // a slot's next bin position, set when its tile is inserted
+        for (int i = threadIdx.x; i < BIN_HASH; i += BLOCK) keys[i] = BIN_EMPTY;
         __syncthreads();
         uint32_t n = ws.seg_count[blockIdx.x];
         const uint4* rec = ws.tris + (size_t)blockIdx.x * (BLOCK * FPT);
@@ -747,14 +744,13 @@ __global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* passes, Workspace
         uint32_t info = ws.seg_info[blockIdx.x];
         uint32_t nlist = info & 0xFFFFu, nw = info >> 16;
         const uint2* list = ws.seg_tiles + (size_t)blockIdx.x * ws.seg_stride;
+        // listed tiles are distinct, so an inserted slot is this thread's:
+        // it reserves the tile's range of the bin at once
         for (uint32_t i = threadIdx.x; i < nlist; i += BLOCK) {
             uint2 e = list[i];
             int sl = bin_slot<BITS>(keys, e.x, true);
-            if (sl >= 0) cnt[sl] = e.y;   // listed tiles are distinct: the slot is this thread's
+            if (sl >= 0) cnt[sl] = atomicAdd(&ws.tile_cursor[e.x], e.y);
         }
-        __syncthreads();
-        for (int i = threadIdx.x; i < BIN_HASH; i += BLOCK)
-            if (keys[i] != BIN_EMPTY) cnt[i] = atomicAdd(&ws.tile_cursor[keys[i]], cnt[i]);
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
             uint4 r = rec[i];
