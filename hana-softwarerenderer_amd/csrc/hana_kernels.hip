@@ -766,15 +766,15 @@ __global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* passes, Workspace
                     if (pos < ws.bin_cap) ws.bins[pos] = r.x;
                 }
         }
-        // wide triangles, one at a time, their tiles spread over the workgroup
+        // wide triangles, one per wave at a time, their tiles spread over its lanes
         const uint16_t* wide = ws.seg_wide + (size_t)blockIdx.x * (BLOCK * FPT);
-        for (uint32_t w = 0; w < nw; w++) {
+        for (uint32_t w = threadIdx.x >> 6; w < nw; w += BLOCK / 64) {
             uint4 r = rec[wide[w]];
             const PassDesc& P = passes[r.y];
             int tx0 = (int)(r.z & 0xFFFFu), ty0 = (int)(r.z >> 16);
             int nx = (int)(r.w & 0xFFFFu) - tx0 + 1;
             int nt = nx * ((int)(r.w >> 16) - ty0 + 1);
-            for (int k = threadIdx.x; k < nt; k += BLOCK) {
+            for (int k = threadIdx.x & 63; k < nt; k += 64) {
                 uint32_t pos = atomicAdd(&ws.tile_cursor[P.tile0 + (ty0 + k / nx) * P.tiles_x + tx0 + k % nx], 1u);
                 if (pos < ws.bin_cap) ws.bins[pos] = r.x;
             }
