@@ -85,6 +85,9 @@ constexpr int kRing = 32;
 constexpr int kDrawSlots = HANA_DRAW_SLOTS;
 // launch sequences of at least this many faces use two of the slots
 constexpr size_t kLargeSequenceFaces = size_t(1) << 19;
+// a draw waits for another slot once that slot has more targets pending for
+// it than this, conflicting or not, which bounds the bookkeeping
+constexpr size_t kMaxPendingTargets = 1024;
 static_assert(kDrawSlots >= 2, "batches run their second part on the stream of draw slot 1");
 
 }  // namespace rt
@@ -118,7 +121,10 @@ struct hana_ctx {
     uint32_t* xstatus_host[hana::rt::kDrawSlots - 1] = {};
     uint32_t* status_dev[hana::rt::kDrawSlots] = {};   // device addresses of the slots' status words
     hipEvent_t slot_done[hana::rt::kDrawSlots] = {};
-    bool slot_pending[hana::rt::kDrawSlots] = {};      // a draw of slot k no other stream has waited for
+    // pend[k][b]: the targets (drawn or sampled) of draws on slot k that slot
+    // b's stream has not waited for; a draw on b waits for slot k only when
+    // it touches one of them
+    std::vector<hana_target*> pend[hana::rt::kDrawSlots][hana::rt::kDrawSlots];
     int next_slot = 0;
     bool pipelined = true;
     // device staging for hana_target_present

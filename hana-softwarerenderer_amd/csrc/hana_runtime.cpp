@@ -48,16 +48,41 @@ uint32_t* slot_status(hana_ctx* c, int k) { return k ? c->xstatus_host[k - 1] : 
 int sync_all(hana_ctx* c) {
     for (int k = 0; k < kDrawSlots; k++)
         if (slot_stream(c, k)) HIP_TRY(c, hipStreamSynchronize(slot_stream(c, k)));
+    for (auto& row : c->pend)
+        for (auto& p : row) p.clear();   // every draw has completed
     return HANA_OK;
 }
 
 int join(hana_ctx* c) {
     for (int k = 1; k < kDrawSlots; k++)
-        if (c->slot_pending[k]) {
+        if (!c->pend[k][0].empty()) {
             HIP_TRY(c, hipStreamWaitEvent(c->stream, c->slot_done[k], 0));
-            c->slot_pending[k] = false;
+            c->pend[k][0].clear();
         }
     return HANA_OK;
+}
+
+// The distinct targets a set of passes draws or samples.
+static std::vector<hana_target*> touched_targets(const std::vector<PassSpec>& specs) {
+    std::vector<hana_target*> t;
+    for (const PassSpec& s : specs)
+        for (hana_target* x : {s.target, s.shadow})
+            if (x && std::find(t.begin(), t.end(), x) == t.end()) t.push_back(x);
+    return t;
+}
+static bool shares_target(const std::vector<hana_target*>& a, const std::vector<hana_target*>& b) {
+    for (hana_target* x : a)
+        if (std::find(b.begin(), b.end(), x) != b.end()) return true;
+    return false;
+}
+// Notes work on slot k's stream, complete at slot_done[k], that touches
+// targets t, for every other slot to wait for before it touches them.
+static void note_pending(hana_ctx* c, int k, const std::vector<hana_target*>& t) {
+    for (int b = 0; b < kDrawSlots; b++)
+        if (b != k)
+            for (hana_target* x : t)
+                if (std::find(c->pend[k][b].begin(), c->pend[k][b].end(), x) == c->pend[k][b].end())
+                    c->pend[k][b].push_back(x);
 }
 
 int ensure(hana_ctx* c, DevBuf& b, size_t bytes) {
@@ -477,6 +502,7 @@ int check_status(hana_ctx* c) {
 // in order.  With timing or statistics on, every call uses slot 0.
 int run_passes(hana_ctx* c, std::vector<PassSpec> specs) {
     int r;
+    const std::vector<hana_target*> touched = touched_targets(specs);
     std::vector<hana_target*> sampled;   // sampled as a shadow map, not drawn here
     for (auto& s : specs) {
         if (!s.shadow || !s.shadow->pending_clear) continue;
@@ -510,10 +536,12 @@ int run_passes(hana_ctx* c, std::vector<PassSpec> specs) {
         HIP_TRY(c, launch_fill32(reinterpret_cast<uint32_t*>(A.stats.p), 0u, (size_t)ST_WORDS * 2 * n, S));
     }
     if ((r = enqueue_geometry(c, pl, A, S, true))) return r;
+    // the raster waits for the calls in flight on other slots that touch one
+    // of its targets; calls on disjoint targets run concurrently
     for (int k = 0; k < kDrawSlots; k++)
-        if (k != a && c->slot_pending[k]) {
+        if (k != a && (c->pend[k][a].size() > kMaxPendingTargets || shares_target(c->pend[k][a], touched))) {
             HIP_TRY(c, hipStreamWaitEvent(S, c->slot_done[k], 0));
-            c->slot_pending[k] = false;
+            c->pend[k][a].clear();
         }
     for (hana_target* t : sampled)
         if ((r = materialize_clear(c, t, S))) return r;
@@ -530,7 +558,7 @@ int run_passes(hana_ctx* c, std::vector<PassSpec> specs) {
                                   hipMemcpyDeviceToHost, S));
         c->stats_passes = n;
     }
-    c->slot_pending[a] = true;
+    if (c->pipelined) note_pending(c, a, touched);   // else no other slot ever draws
     c->next_slot = (a + 1) % nslots;
     c->status_pending = true;
     return HANA_OK;
@@ -1089,11 +1117,17 @@ int hana_batch_run(hana_ctx* c, hana_batch* b) {
     int r = join(c);
     if (r) return r;
     c->next_slot = 0;   // the next draw follows the sequence on the context stream
-    if (b->use_graph) {
-        HIP_TRY(c, hipGraphLaunch(b->exec, c->stream));
-        return HANA_OK;
-    }
-    return batch_enqueue(c, b, true);
+    if (b->use_graph) HIP_TRY(c, hipGraphLaunch(b->exec, c->stream));
+    else if ((r = batch_enqueue(c, b, true))) return r;
+    // every part has joined the context stream: draws on other slots that
+    // touch the batch's targets wait for it
+    HIP_TRY(c, hipEventRecord(c->slot_done[0], c->stream));
+    std::vector<hana_target*> t;
+    for (const PassSpec& s : b->specs)
+        for (hana_target* x : {s.target, s.shadow})
+            if (x) t.push_back(x);
+    note_pending(c, 0, t);
+    return HANA_OK;
 }
 
 int hana_batch_destroy(hana_ctx* c, hana_batch* b) {

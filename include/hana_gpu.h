@@ -26,9 +26,10 @@
  * device object created on it; calls on one context must be externally
  * synchronised.  Work is enqueued asynchronously: consecutive draws rotate
  * over the streams (over two of them for draws of 2^19 faces or more) so that
- * a draw's geometry stages overlap the rasters of the draws before it, and
- * every raster waits for the previous draw, so targets see the draws in call
- * order.  hana_target_read(_winner),
+ * a draw's geometry stages overlap the rasters of the draws before it.  A
+ * raster waits for every earlier draw that draws or samples one of its
+ * targets or shadow maps, so each target sees the draws in call order; draws
+ * on disjoint targets may run concurrently.  hana_target_read(_winner),
  * hana_target_present and hana_gpu_sync wait for all of it.  If a draw since
  * the previous such wait outgrew the workspace, the wait returns
  * HANA_E_OVERFLOW after growing it: render those frames again.

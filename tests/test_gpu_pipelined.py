@@ -4,8 +4,10 @@ own workspaces, so a draw's geometry stages overlap the rasters before it;
 the results must still be those of the calls run one after another:
 depth tests across draws, shadow maps written by one call and sampled by the
 next, targets read, uploaded or presented between calls, timing switched
-on in the middle of a chain, and chains that mix calls of at least 2^19 faces
-(which rotate over two slots) with smaller ones (three slots)."""
+on in the middle of a chain, chains that mix calls of at least 2^19 faces
+(which rotate over two slots) with smaller ones (three slots), and calls on
+several targets interleaved, which wait only for the calls that share a
+target or shadow map with them."""
 import numpy as np
 import pytest
 
@@ -145,3 +147,45 @@ def test_large_and_small_calls_mixed(ctx, oracle, wh):
     ref = oracle_chain(oracle, order)
     assert_depth_exact(dep, ref.depth, "mixed")
     assert_color_ok(col, ref.color, "mixed")
+
+
+def test_interleaved_targets(ctx, oracle, wh):
+    """28 calls in seeded random order over four targets: plain draws, and
+    shadow passes into one target sampled by a main pass into another.  Calls
+    on disjoint targets may run concurrently; every target still equals the
+    oracle running the calls one after another, and so does a read in the
+    middle."""
+    from hana_amd.gpu import SHADOW, build_uniforms
+    W, H = wh
+    rng = np.random.default_rng(77)
+    kinds = [sg.BLINN, sg.NORMALMAP, sg.TOON, sg.TEXTURE]
+    scenes = [sg.small_scene(kinds[i % 4], shadow=True, seed=350 + i, width=W, height=H,
+                             cam=[(0.0, 0.0, 2.0), (0.4, 0.3, 1.8), (-0.5, 0.1, 1.6)][i % 3]) for i in range(6)]
+    devm = [ctx.mesh(sc.mesh) for sc in scenes]
+    devt = [(ctx.texture(sc.diffuse), ctx.texture(sc.normal)) for sc in scenes]
+    tg = [ctx.target(W, H) for _ in range(4)]
+    ref = [HostTarget(W, H) for _ in range(4)]
+    keep = []
+    for step in range(28):
+        i = int(rng.integers(6))
+        sc = scenes[i]
+        u = build_uniforms(sc)
+        ou = oracle.build_uniforms(sc)
+        keep.append(u)
+        j = int(rng.integers(4))
+        if rng.random() < 0.35:
+            m = int((j + 1 + rng.integers(3)) % 4)        # main pass into another target, sampling j
+            ctx.draw(devm[i], SHADOW, u, tg[j], *devt[i])
+            ctx.draw(devm[i], sc.shader, u, tg[m], *devt[i], shadow_map=tg[j])
+            oracle.draw(sc.mesh, sg.SHADOW, ou, ref[j], sc.diffuse, sc.normal)
+            oracle.draw(sc.mesh, sc.shader, ou, ref[m], sc.diffuse, sc.normal, ref[j])
+        else:
+            ctx.draw(devm[i], sc.shader, u, tg[j], *devt[i])
+            oracle.draw(sc.mesh, sc.shader, ou, ref[j], sc.diffuse, sc.normal)
+        if step == 13:
+            col, dep = ctx.read(tg[2], W, H)
+            assert_depth_exact(dep, ref[2].depth, "middle")
+    for k in range(4):
+        col, dep = ctx.read(tg[k], W, H)
+        assert_depth_exact(dep, ref[k].depth, f"target {k}")
+        assert_color_ok(col, ref[k].color, f"target {k}")
