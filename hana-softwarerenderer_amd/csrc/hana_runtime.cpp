@@ -549,6 +549,7 @@ int run_passes(hana_ctx* c, std::vector<PassSpec> specs) {
     HIP_TRY(c, hipEventRecord(c->slot_done[a], S));
     if (c->stats_enabled) {
         if (c->stats_passes < n) {
+            HIP_TRY(c, hipStreamSynchronize(S));   // the previous call may still copy into it
             if (c->stats_host) HIP_TRY(c, hipHostFree(c->stats_host));
             c->stats_host = nullptr;
             c->stats_passes = 0;
