@@ -80,7 +80,7 @@ struct Plan {
 constexpr int kRing = 32;
 // streams (each with a workspace) that consecutive per-call draws rotate over
 #ifndef HANA_DRAW_SLOTS
-#define HANA_DRAW_SLOTS 3
+#define HANA_DRAW_SLOTS 4
 #endif
 constexpr int kDrawSlots = HANA_DRAW_SLOTS;
 // launch sequences of at least this many faces use two of the slots
@@ -88,7 +88,7 @@ constexpr size_t kLargeSequenceFaces = size_t(1) << 19;
 // a draw waits for another slot once that slot has more targets pending for
 // it than this, conflicting or not, which bounds the bookkeeping
 constexpr size_t kMaxPendingTargets = 1024;
-static_assert(kDrawSlots >= 2, "batches run their second part on the stream of draw slot 1");
+static_assert(kDrawSlots >= 1, "a context has at least its own stream");
 
 }  // namespace rt
 }  // namespace hana
@@ -96,6 +96,7 @@ static_assert(kDrawSlots >= 2, "batches run their second part on the stream of d
 struct hana_ctx {
     using Arena = hana::rt::Arena;
     int device = 0;
+    int nslots = 1;   // draw slots of this context (at most kDrawSlots)
     hipStream_t stream = nullptr;
     std::string err;
     bool stats_enabled = false;
@@ -111,7 +112,7 @@ struct hana_ctx {
     uint64_t* stats_host = nullptr;    // pinned, per pass
     int stats_passes = 0;
     bool status_pending = false;
-    // pipelined draws: consecutive draws rotate over kDrawSlots slots, each a
+    // pipelined draws: consecutive draws rotate over nslots slots, each a
     // stream with its own workspace and status words (slot 0 is (stream,
     // arena, status_host), slot k > 0 is (xstream, xarena, xstatus_host)[k-1]),
     // so the geometry stages of a draw overlap the rasters of the draws before;
@@ -165,6 +166,8 @@ int fail(hana_ctx* c, int code, const std::string& msg);
 int hip_fail(hana_ctx* c, hipError_t e, const char* what);
 int ensure(hana_ctx* c, DevBuf& b, size_t bytes);
 int materialize_clear(hana_ctx* c, hana_target* t, hipStream_t s);
+// A context with nslots draw slots (1..kDrawSlots); hana_gpu_create makes kDrawSlots.
+int create_context(int device, int nslots, hana_ctx** out);
 // Orders the context stream after every draw in flight on another slot's stream.
 int join(hana_ctx* c);
 // Waits for every stream of the context.

@@ -46,7 +46,7 @@ Arena& slot_arena(hana_ctx* c, int k) { return k ? c->xarena[k - 1] : c->arena; 
 uint32_t* slot_status(hana_ctx* c, int k) { return k ? c->xstatus_host[k - 1] : c->status_host; }
 
 int sync_all(hana_ctx* c) {
-    for (int k = 0; k < kDrawSlots; k++)
+    for (int k = 0; k < c->nslots; k++)
         if (slot_stream(c, k)) HIP_TRY(c, hipStreamSynchronize(slot_stream(c, k)));
     for (auto& row : c->pend)
         for (auto& p : row) p.clear();   // every draw has completed
@@ -54,7 +54,7 @@ int sync_all(hana_ctx* c) {
 }
 
 int join(hana_ctx* c) {
-    for (int k = 1; k < kDrawSlots; k++)
+    for (int k = 1; k < c->nslots; k++)
         if (!c->pend[k][0].empty()) {
             HIP_TRY(c, hipStreamWaitEvent(c->stream, c->slot_done[k], 0));
             c->pend[k][0].clear();
@@ -78,7 +78,7 @@ static bool shares_target(const std::vector<hana_target*>& a, const std::vector<
 // Notes work on slot k's stream, complete at slot_done[k], that touches
 // targets t, for every other slot to wait for before it touches them.
 static void note_pending(hana_ctx* c, int k, const std::vector<hana_target*>& t) {
-    for (int b = 0; b < kDrawSlots; b++)
+    for (int b = 0; b < c->nslots; b++)
         if (b != k)
             for (hana_target* x : t)
                 if (std::find(c->pend[k][b].begin(), c->pend[k][b].end(), x) == c->pend[k][b].end())
@@ -462,7 +462,7 @@ int check_status(hana_ctx* c) {
     if (r) return r;
     c->status_pending = false;
     uint32_t flags = 0, need_bins = 0, need_polys = 0, need_tris = 0;
-    for (int k = 0; k < kDrawSlots; k++) {
+    for (int k = 0; k < c->nslots; k++) {
         const uint32_t* host = slot_status(c, k);
         if (!host) continue;
         flags |= host[STATUS_FLAGS];
@@ -471,7 +471,7 @@ int check_status(hana_ctx* c) {
         need_tris = std::max(need_tris, host[STATUS_NEED_TRIS]);
     }
     if (!flags) return HANA_OK;
-    for (int k = 0; k < kDrawSlots; k++) {
+    for (int k = 0; k < c->nslots; k++) {
         uint32_t* host = slot_status(c, k);
         if (!host) continue;
         Arena& A = slot_arena(c, k);
@@ -489,14 +489,14 @@ int check_status(hana_ctx* c) {
                     "a draw, frame or batch part covered more than 4,294,967,280 (triangle, tile) pairs and was not "
                     "rasterised; split it");
     // grow for the next call; the frames since the last check must be re-rendered
-    for (int k = 0; k < kDrawSlots; k++)
+    for (int k = 0; k < c->nslots; k++)
         if ((r = grow_arena(c, slot_arena(c, k), flags, need_bins, need_polys, need_tris))) return r;
     return fail(c, HANA_E_OVERFLOW,
                 "bin or clip-pool capacity exceeded since the last sync; buffers were grown, re-render the frame");
 }
 
 // Renders a list of passes (per-call path).  Consecutive calls rotate over
-// kDrawSlots slots, each a stream with its own workspace: the geometry
+// the context's slots, each a stream with its own workspace: the geometry
 // stages of a call run while earlier calls rasterise, and its raster waits
 // for the previous call to finish, so targets and shadow maps see the calls
 // in order.  With timing or statistics on, every call uses slot 0.
@@ -516,7 +516,7 @@ int run_passes(hana_ctx* c, std::vector<PassSpec> specs) {
     // Large sequences rotate over two slots: with a third in flight, the 2.6 M
     // faces of a 1.3 M-triangle frame with its shadow pass ran 1.2 % slower,
     // while frames of up to 400 k faces ran 6 % faster.
-    int nslots = pl.total_f >= kLargeSequenceFaces ? 2 : kDrawSlots;
+    int nslots = pl.total_f >= kLargeSequenceFaces ? std::min(2, c->nslots) : c->nslots;
     int a = rotate ? c->next_slot % nslots : 0;
     hipStream_t S = slot_stream(c, a);
     Arena& A = slot_arena(c, a);
@@ -538,7 +538,7 @@ int run_passes(hana_ctx* c, std::vector<PassSpec> specs) {
     if ((r = enqueue_geometry(c, pl, A, S, true))) return r;
     // the raster waits for the calls in flight on other slots that touch one
     // of its targets; calls on disjoint targets run concurrently
-    for (int k = 0; k < kDrawSlots; k++)
+    for (int k = 0; k < c->nslots; k++)
         if (k != a && (c->pend[k][a].size() > kMaxPendingTargets || shares_target(c->pend[k][a], touched))) {
             HIP_TRY(c, hipStreamWaitEvent(S, c->slot_done[k], 0));
             c->pend[k][a].clear();
@@ -576,18 +576,26 @@ bool valid_kind(int k) { return k >= HANA_SHADER_GROUND && k <= HANA_SHADER_SHAD
 
 extern "C" {
 
-int hana_gpu_create(int device, hana_ctx** out) {
-    if (!out) return HANA_E_INVALID;
+int hana_gpu_create(int device, hana_ctx** out) { return create_context(device, kDrawSlots, out); }
+
+}  // extern "C"
+
+namespace hana {
+namespace rt {
+
+int create_context(int device, int nslots, hana_ctx** out) {
+    if (!out || nslots < 1 || nslots > kDrawSlots) return HANA_E_INVALID;
     *out = nullptr;
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return HANA_E_DEVICE;
     if (device < 0 || device >= count) return HANA_E_INVALID;
     hana_ctx* c = new hana_ctx();
     c->device = device;
+    c->nslots = nslots;
     auto undo = [c]() {
         for (int i = 0; i < kRing; i++)
             if (c->ring_event[i]) (void)hipEventDestroy(c->ring_event[i]);
-        for (int k = 0; k < kDrawSlots; k++) {
+        for (int k = 0; k < c->nslots; k++) {
             if (c->slot_done[k]) (void)hipEventDestroy(c->slot_done[k]);
             if (slot_status(c, k)) (void)hipHostFree(slot_status(c, k));
             if (slot_stream(c, k)) (void)hipStreamDestroy(slot_stream(c, k));
@@ -600,7 +608,7 @@ int hana_gpu_create(int device, hana_ctx** out) {
         return undo();
     for (int i = 0; i < kRing; i++)
         if (hipEventCreateWithFlags(&c->ring_event[i], hipEventDisableTiming) != hipSuccess) return undo();
-    for (int k = 0; k < kDrawSlots; k++) {
+    for (int k = 0; k < c->nslots; k++) {
         if (k && hipStreamCreateWithFlags(&c->xstream[k - 1], hipStreamNonBlocking) != hipSuccess) return undo();
         uint32_t*& host = k ? c->xstatus_host[k - 1] : c->status_host;
         if (hipHostMalloc((void**)&host, sizeof(uint32_t) * STATUS_WORDS) != hipSuccess) return undo();
@@ -611,6 +619,11 @@ int hana_gpu_create(int device, hana_ctx** out) {
     *out = c;
     return HANA_OK;
 }
+
+}  // namespace rt
+}  // namespace hana
+
+extern "C" {
 
 static void free_arena(Arena& a) {
     DevBuf* bufs[] = {&a.tables, &a.vr, &a.vs, &a.code, &a.tris, &a.seg_count, &a.seg_tiles, &a.seg_wide, &a.seg_info, &a.scan_part, &a.pof, &a.tile_count, &a.tile_offset,
@@ -627,7 +640,7 @@ static void free_arena(Arena& a) {
 int hana_gpu_destroy(hana_ctx* c) {
     if (!c) return HANA_E_INVALID;
     (void)hipSetDevice(c->device);
-    for (int k = 0; k < kDrawSlots; k++) {
+    for (int k = 0; k < c->nslots; k++) {
         (void)hipStreamSynchronize(slot_stream(c, k));
         free_arena(slot_arena(c, k));
     }
@@ -637,7 +650,7 @@ int hana_gpu_destroy(hana_ctx* c) {
         if (c->ring_event[i]) (void)hipEventDestroy(c->ring_event[i]);
     }
     if (c->stats_host) (void)hipHostFree(c->stats_host);
-    for (int k = 0; k < kDrawSlots; k++) {
+    for (int k = 0; k < c->nslots; k++) {
         if (slot_status(c, k)) (void)hipHostFree(slot_status(c, k));
         if (c->slot_done[k]) (void)hipEventDestroy(c->slot_done[k]);
     }
@@ -646,7 +659,7 @@ int hana_gpu_destroy(hana_ctx* c) {
         (void)hipEventDestroy(p.second.second);
     }
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
-    for (int k = 0; k < kDrawSlots; k++) (void)hipStreamDestroy(slot_stream(c, k));
+    for (int k = 0; k < c->nslots; k++) (void)hipStreamDestroy(slot_stream(c, k));
     delete c;
     return HANA_OK;
 }
@@ -1011,7 +1024,7 @@ static void batch_split(hana_batch* b) {
 // context stream), and parts beyond the slots on the batch's own streams, so
 // that a context and its batches use as few streams as possible.
 static hipStream_t part_stream(hana_ctx* c, hana_batch* b, size_t p) {
-    return p < (size_t)kDrawSlots ? slot_stream(c, (int)p) : b->extra[p - kDrawSlots];
+    return p < (size_t)c->nslots ? slot_stream(c, (int)p) : b->extra[p - c->nslots];
 }
 
 // Enqueues every part on its stream, forked from and joined back into the
@@ -1089,7 +1102,7 @@ int hana_batch_finalize(hana_ctx* c, hana_batch* b, int use_graph) {
         HIP_TRY(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
         b->joined.push_back(e);
     }
-    while (b->extra.size() + kDrawSlots < b->parts.size()) {
+    while (b->extra.size() + c->nslots < b->parts.size()) {
         hipStream_t s = nullptr;
         HIP_TRY(c, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         b->extra.push_back(s);

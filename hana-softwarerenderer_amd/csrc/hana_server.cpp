@@ -309,7 +309,11 @@ int hana_server_create(hana_ctx* ctx, const hana_server_config* cfg, hana_server
     s->cfg = *cfg;
     for (int i = 0; i < cfg->lanes; i++) {
         Lane L;
-        if (hana_gpu_create(ctx->device, &L.ctx) != HANA_OK ||
+        // Lane contexts hold three streams though a lane draws on one: lanes
+        // of four-stream contexts served C4 with p99 2.5 ms, three-stream ones
+        // with 1.0 ms; likely because streams map round-robin onto the four
+        // hardware queues of a process, so the lanes' own streams then share one.
+        if (create_context(ctx->device, 3, &L.ctx) != HANA_OK ||
             hipEventCreateWithFlags(&L.done, hipEventDisableTiming) != hipSuccess) {
             if (L.ctx) hana_gpu_destroy(L.ctx);
             for (Lane& l : s->lanes) {

@@ -22,7 +22,7 @@
  *   hana_scene_*         Scene / SingleModelScene state, orbit      SR/scene.cpp:3-9,45-63,127-147,
  *                        camera and its reset                       SR/camera.cpp:31-70,167-169
  *
- * A context owns three HIP streams, each with its own workspace, and every
+ * A context owns four HIP streams, each with its own workspace, and every
  * device object created on it; calls on one context must be externally
  * synchronised.  Work is enqueued asynchronously: consecutive draws rotate
  * over the streams (over two of them for draws of 2^19 faces or more) so that
