@@ -267,6 +267,34 @@ def test_widest_target(ctx, oracle):
     assert_color_ok(col, ref.color, "widest")
 
 
+def test_pass_statistics(ctx, oracle):
+    """hana_gpu_last_stats of a shadow pass and a main pass with clipping:
+    faces and faces failing the trivial accept equal the oracle's counts,
+    pixels written equal the pixels the oracle's fragments reached, and every
+    winning primitive has a bin entry."""
+    from hana_amd.gpu import SHADOW, build_uniforms
+    sc = sg.small_scene(sg.BLINN, shadow=True, seed=7, cam=(0.3, 0.2, 1.2))
+    tgt_o, owin, osm, st, st_s = oracle_passes(oracle, sc)
+    u = build_uniforms(sc)
+    mesh, dif, nrm = ctx.mesh(sc.mesh), ctx.texture(sc.diffuse), ctx.texture(sc.normal)
+    sw, sh = sc.shadow_wh
+    sm, tgt = ctx.target(sw, sh), ctx.target(sc.width, sc.height)
+    ctx.set_stats(True)
+    try:
+        ctx.draw(mesh, SHADOW, u, sm, dif, nrm)
+        (gs,) = ctx.last_stats()
+        ctx.draw(mesh, sc.shader, u, tgt, dif, nrm, sm)
+        (gm,) = ctx.last_stats()
+    finally:
+        ctx.set_stats(False)
+    for g, o, win in ((gs, st_s, osm.winner), (gm, st, owin)):
+        assert g["faces"] == o["faces"] and g["clipped_faces"] == o["clipped_faces"], (g, o)
+        assert g["polygons"] <= g["clipped_faces"]
+        assert g["pixels_written"] == int((win >= 0).sum())
+        assert g["tile_entries"] >= len(np.unique(win[win >= 0]))
+    assert st["clipped_faces"] > 0 and gm["polygons"] > 0
+
+
 def test_nan_depth_fragments(ctx, oracle):
     """Fragments of NaN depth (a vertex whose clip z and w overflow to inf):
     they replace any stored depth, every later fragment at their pixels
