@@ -297,6 +297,23 @@ def measured_traffic(stage):
     return None
 
 
+def measured_valu(stage):
+    """VALU wave instructions per launch of a stage (SQ_INSTS_VALU) from the
+    committed counter summary of this workload (None if absent)."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    per = json.load(open(TRAFFIC_FILE)).get("per_launch", {})
+    for k, v in per.items():
+        if k.split(" ")[0] == stage and "valu_instr" in v:
+            return int(v["valu_instr"])
+    return None
+
+
+# VALU issue peak of MI355X: 256 CUs x 4 SIMDs at 2.4 GHz, one wave64 FP32
+# instruction per SIMD every 4 cycles (16 lanes a cycle; packed FP32 is off).
+VALU_PEAK_PER_S = 256 * 4 * 2.4e9 / 4
+
+
 def alg_bytes(scene):
     """Algorithmic HBM bytes of one frame, split by stage:
     a2v 32 B and indices 12 B per vertex/triangle per pass, colour+depth 8 B
@@ -434,6 +451,14 @@ def main():
                      "frame_achieved": round(frame_bytes / (ms_per_step * 1e-3) / 1e9, 2)},
         "parity": parity,
     }
+    valu = measured_valu(dom)
+    if valu is not None and stage_ms[dom] > 0:
+        rate = valu / (stage_ms[dom] * 1e-3)
+        res["valu"] = {"kernel": dom, "wave_instr_per_launch": valu, "achieved": round(rate / 1e9, 2),
+                       "peak": round(VALU_PEAK_PER_S / 1e9, 1), "unit": "G wave64 VALU instr/s",
+                       "frac": round(rate / VALU_PEAK_PER_S, 4),
+                       "source": "SQ_INSTS_VALU per dispatch from " + os.path.relpath(TRAFFIC_FILE, REPO) +
+                                 " over the launch time above: the bound of this kernel"}
     if cpu is not None:
         cpu_mfrag = cpu["fragments"] / cpu["sec_per_frame"] / 1e6
         res["cpu_baseline"] = {"value": round(cpu_mfrag, 4), "unit": "Mfrag/s", "cores": 1, "kind": cpu["kind"],
