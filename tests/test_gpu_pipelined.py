@@ -5,7 +5,7 @@ the results must still be those of the calls run one after another:
 depth tests across draws, shadow maps written by one call and sampled by the
 next, targets read, uploaded or presented between calls, timing switched
 on in the middle of a chain, chains that mix calls of at least 2^19 faces
-(which rotate over two slots) with smaller ones (three slots), and calls on
+(which rotate over two slots) with smaller ones (all four), and calls on
 several targets interleaved, which wait only for the calls that share a
 target or shadow map with them."""
 import numpy as np
@@ -129,7 +129,7 @@ def test_reads_uploads_and_presents_between_draws(ctx, oracle, wh):
 
 def test_large_and_small_calls_mixed(ctx, oracle, wh):
     """Calls of 588 k faces (two slots) between calls of a few thousand
-    (three slots), into one target: every raster waits for the call before
+    (all four slots), into one target: every raster waits for the call before
     it whatever slots the two use."""
     from hana_amd.gpu import build_uniforms
     W, H = wh
