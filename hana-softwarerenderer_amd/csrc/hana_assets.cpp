@@ -233,10 +233,11 @@ int hana_obj_load(const char* path, hana_obj_mesh* out) {
         hana_obj_free(out);
         return asset_fail(HANA_E_NOMEM, "hana_obj_load: out of host memory");
     }
-    std::memcpy(out->pos, pos.data(), sizeof(float) * pos.size());
-    std::memcpy(out->nrm, nrm.data(), sizeof(float) * nrm.size());
-    std::memcpy(out->uv, uv.data(), sizeof(float) * uv.size());
-    std::memcpy(out->idx, idx.data(), sizeof(int32_t) * idx.size());
+    // an empty list's data() may be null, which memcpy does not accept
+    if (!pos.empty()) std::memcpy(out->pos, pos.data(), sizeof(float) * pos.size());
+    if (!nrm.empty()) std::memcpy(out->nrm, nrm.data(), sizeof(float) * nrm.size());
+    if (!uv.empty()) std::memcpy(out->uv, uv.data(), sizeof(float) * uv.size());
+    if (!idx.empty()) std::memcpy(out->idx, idx.data(), sizeof(int32_t) * idx.size());
     return HANA_OK;
 }
 
