@@ -149,9 +149,12 @@ __device__ __forceinline__ void vertex_program(int kind, const DevUniforms& u, F
 }
 
 // IShader::is_in_shadow (SR/IShader.h:107-129).  The shadow map is an RGBA8
-// render target whose R byte holds the quantised light-space depth.
+// render target whose R byte holds the quantised light-space depth; in the
+// fp32 extension its fp32 depth buffer d gives the nearest caster's light clip
+// z as 2*d - 1.
 struct ShadowMap {
     const uint32_t* color;
+    const float* depth;
     int w, h;
 };
 __device__ __forceinline__ int is_in_shadow(const DevUniforms& u, const ShadowMap& sm, const float dp[4], float n_dot_l) {
@@ -166,8 +169,10 @@ __device__ __forceinline__ int is_in_shadow(const DevUniforms& u, const ShadowMa
         float current_depth = dp[2] - depth_bias;
         if (posx < 0 || posy < 0 || posx >= widht || posy >= height) return 1;
         int px = (int)posx, py = (int)posy;
-        uint32_t c = sm.color[(size_t)py * (size_t)sm.w + (size_t)px];
-        float closest_depth = (float)(c & 0xFF) / 255.f;
+        size_t texel = (size_t)py * (size_t)sm.w + (size_t)px;
+        float closest_depth;
+        if (u.shadow_fp32) closest_depth = 2.f * sm.depth[texel] - 1.f;
+        else closest_depth = (float)(sm.color[texel] & 0xFF) / 255.f;
         return current_depth < closest_depth;
     }
     return 1;

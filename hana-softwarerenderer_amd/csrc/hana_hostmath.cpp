@@ -194,6 +194,7 @@ void derive_dev_uniforms(const hana_uniforms& u, DevUniforms& d) {
     d.gloss = u.gloss;
     d.bump_scale = u.bump_scale;
     d.enable_shadow = u.enable_shadow;
+    d.shadow_fp32 = u.shadow_fp32 ? 1 : 0;
 }
 
 }  // namespace hana

@@ -51,12 +51,20 @@ struct PassSpec {
     int group;              // raster dependency group (shadow passes first)
 };
 
+// How a launch sequence is laid out: tile size, raster form, batch parts
+// (hana_tuning of the C ABI).
+struct Tuning {
+    int tile_shift = TILE_SHIFT_DEFAULT;   // tiles of 2^shift pixels a side
+    int raster_wgs_per_cu = 0;            // 0: one raster workgroup per tile; else a persistent raster
+    int parts = 4;                         // concurrent parts of a batch
+};
+
 // Grow-only device workspace for launch sequences.
 struct Arena {
     // tables: the pass descriptors, then (at Plan::uni_off) their uniforms,
     // uploaded by one copy per call
     DevBuf tables, vr, vs, code, pof, tile_count, tile_offset, tile_cursor, bins, tris, seg_count, scan_part;
-    DevBuf seg_tiles, seg_wide, seg_info;
+    DevBuf seg_tiles, seg_wide, seg_info, qstate;
     DevBuf pool_vr, pool_vs, pool_face, pool_pass, pool_n, clip_queue, status, stats;
     uint32_t bin_cap = 0, poly_cap = 0, clipq_cap = 0, tris_cap = 0;
 };
@@ -74,10 +82,31 @@ struct Plan {
     size_t uni_off = 0;   // byte offset of the uniforms in the tables buffer
     int total_tiles = 0;
     bool small_tris = true;   // every pass has at most 4 target pixels per face
+    int tile_shift = TILE_SHIFT_DEFAULT;
+    int raster_wgs = 0;       // workgroups of the persistent raster (0: one per tile and group)
+    uint32_t q_g0 = 0, q_dep_from = 0;
     Workspace ws{};
 };
 
+// A clear fused into nothing: a shadow map sampled by a call while a clear of
+// it was pending is filled with the clear values before that call's raster.
+struct Fill {
+    hana_target* target;
+    uint32_t rgba;
+    float depth;
+};
+
+// A per-call launch sequence as enqueued, kept until the next status check
+// so that it can be rendered again if a workspace capacity ran out.
+struct LoggedCall {
+    Plan pl;
+    std::vector<Fill> fills;
+};
+
 constexpr int kRing = 32;
+// calls kept for replay; past this many a call first checks the status of
+// those before it (a sync)
+constexpr size_t kMaxLoggedCalls = 1024;
 // streams (each with a workspace) that consecutive per-call draws rotate over
 #ifndef HANA_DRAW_SLOTS
 #define HANA_DRAW_SLOTS 4
@@ -96,6 +125,8 @@ static_assert(kDrawSlots >= 1, "a context has at least its own stream");
 struct hana_ctx {
     using Arena = hana::rt::Arena;
     int device = 0;
+    int num_cus = 256;
+    hana::rt::Tuning tuning;   // of per-call draws
     int nslots = 1;   // draw slots of this context (at most kDrawSlots)
     hipStream_t stream = nullptr;
     std::string err;
@@ -134,22 +165,33 @@ struct hana_ctx {
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+    // per-call sequences since the last status check, in call order; a check
+    // that finds a capacity overflow grows the workspaces and renders them
+    // again (replay_overflow), else reports HANA_E_OVERFLOW
+    std::vector<hana::rt::LoggedCall> log;
+    bool replay_overflow = true;
+    int64_t replays = 0;   // status checks that rendered the logged calls again
 };
 
-// A batch's frames are split into up to kBatchParts parts, each a launch
+// A batch's frames are split into up to kMaxBatchParts parts, each a launch
 // sequence with its own workspace; the parts run concurrently, part p on the
 // stream of draw slot p and parts beyond the slots on streams of the batch
 // (branches of one graph), since their frames share no target.
+constexpr size_t kMaxBatchParts = 4;
+
 struct hana_batch_part {
     std::vector<hana::rt::PassSpec> specs;
     hana::rt::Plan plan;
-    hana::rt::Arena arena;
+    hana::rt::Arena* arena = nullptr;   // the batch's arena of the same index
 };
 
 struct hana_batch {
     std::vector<hana::rt::PassSpec> specs;
     std::vector<size_t> frame_first;   // index in specs of each frame's first pass
     std::vector<hana_batch_part> parts;
+    hana::rt::Arena arenas[kMaxBatchParts];   // grow-only, kept across tunings
+    hana::rt::Tuning tuning;
+    double tuned_ms = 0.0;                    // run time the autotuner measured for `tuning`
     bool finalized = false;
     bool use_graph = false;
     hipGraph_t graph = nullptr;
@@ -179,6 +221,8 @@ uint32_t* slot_status(hana_ctx* c, int k);
 int check_status(hana_ctx* c);
 // Enqueues the launch sequence of a set of passes on the context's stream.
 int run_passes(hana_ctx* c, std::vector<PassSpec> specs);
+// Validates a hana_tuning into a Tuning.
+int tuning_from(hana_ctx* c, const hana_tuning* t, Tuning& out);
 bool valid_kind(int k);
 
 }  // namespace rt

@@ -168,6 +168,12 @@ __device__ __forceinline__ void stat_add(const PassDesc& P, int which, uint64_t 
     if (P.stats && v) atomicAdd((unsigned long long*)&P.stats[which], (unsigned long long)v);
 }
 
+// Raises a flag for this sequence and in the sticky word the host reads.
+__device__ __forceinline__ void raise_flag(const Workspace& ws, uint32_t f) {
+    atomicOr(&ws.status[STATUS_SEQ_FLAGS], f);
+    atomicOr(&ws.status[STATUS_FLAGS], f);
+}
+
 // clip planes in clip_triangle's order (SR/graphics.cpp:136-161) and their
 // inside tests (is_inside_plane, SR/graphics.cpp:29-49)
 enum { PL_W, PL_PX, PL_NX, PL_PY, PL_NY, PL_PZ, PL_NZ };
@@ -193,6 +199,7 @@ __global__ __launch_bounds__(BLOCK) void k_vertex(const PassDesc* passes, int np
     int gsize = gridDim.x * BLOCK;
     for (int i = gtid; i < ws.ntiles; i += gsize) ws.tile_count[i] = 0;
     if (gtid < STATUS_SEQ_WORDS) ws.status[gtid] = 0;
+    if (gtid < 2) ws.qstate[gtid] = 0;
 
     const PassDesc& P = passes[find_pass<0>(passes, npasses, blockIdx.x)];
     const DevUniforms& u = *P.u;
@@ -278,10 +285,11 @@ __device__ __forceinline__ void wave_sync() {
 // of the group owning vertex i of the polygon.  Its two polygon buffers
 // (clip position then POLY_VARY varyings per vertex) live in LDS.
 constexpr int CLIP_GROUP = 16;
-constexpr int CLIP_THREADS = 1024;
+constexpr int CLIP_THREADS = 512;
 constexpr int CLIP_FACES = CLIP_THREADS / CLIP_GROUP;   // faces per workgroup and round
 constexpr int CLIP_NF = 4 + POLY_VARY;
 static_assert(POLY_MAX <= CLIP_GROUP, "one lane per polygon vertex");
+static_assert(POLY_MAX - 2 <= (1 << FAN_BITS), "fan index fits its bin-entry field");
 
 // The homogeneous divide of a clip-space vertex, as the face loop applies it
 // to the clipped vertices (SR/graphics.cpp:317, 336).
@@ -394,15 +402,15 @@ struct SegTables {       // k_setup's LDS tables of one segment
     uint32_t* nwide;
 };
 
-template <int BITS>
+template <int BITS, int TS>
 __device__ __forceinline__ void count_tiles(const PassDesc& P, const Workspace& ws, int pass, bool ok, const TriS& t,
                                             uint32_t entry, uint32_t slot, uint32_t local, const SegTables* L) {
     int tx0 = 0, ty0 = 0, tx1 = 0, ty1 = 0, nx = 0, n = 0;
     if (ok) {
-        tx0 = t.x0 >> TILE_SHIFT;
-        ty0 = t.y0 >> TILE_SHIFT;
-        tx1 = t.x1 >> TILE_SHIFT;
-        ty1 = t.y1 >> TILE_SHIFT;
+        tx0 = t.x0 >> TS;
+        ty0 = t.y0 >> TS;
+        tx1 = t.x1 >> TS;
+        ty1 = t.y1 >> TS;
         nx = tx1 - tx0 + 1;
         n = nx * (ty1 - ty0 + 1);
         ws.tris[slot] = make_uint4(entry, (uint32_t)pass, (uint32_t)tx0 | ((uint32_t)ty0 << 16),
@@ -468,7 +476,7 @@ __device__ __forceinline__ bool dies_at_first_crossed_plane(uint32_t c0, uint32_
 // (BLOCK * FPT records from blockIdx.x * BLOCK * FPT; the count goes to
 // ws.seg_count); faces that clipping would remove are dropped; the others are
 // queued for k_clip with one global atomic per workgroup.
-template <int BITS>
+template <int BITS, int TS>
 __global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* passes, int npasses, Workspace ws) {
     constexpr int SLOTS = 1 << BITS;
     __shared__ uint64_t s_queue[BLOCK * FPT];
@@ -533,7 +541,7 @@ __global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* passes, int npa
         TriS t;
         bool ok = trivial[k] && tri_setup(va[k], vb[k], vc[k], P.width, P.height, t);
         uint32_t slot = wave_reserve_lds(&s_ntri, ok);
-        count_tiles<BITS>(P, ws, pass, ok, t, (uint32_t)f, seg + slot, slot, &L);
+        count_tiles<BITS, TS>(P, ws, pass, ok, t, (uint32_t)f, seg + slot, slot, &L);
     }
     __syncthreads();
     // add the segment's counts to the global ones, and hand k_bin the list of
@@ -555,7 +563,7 @@ __global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* passes, int npa
     for (uint32_t i = threadIdx.x; i < s_nq; i += BLOCK) {
         uint32_t q = s_qbase + i;
         if (q < ws.clipq_cap) ws.clip_queue[q] = s_queue[i];
-        else atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_CLIPQ_OVERFLOW);
+        else raise_flag(ws, FLAG_CLIPQ_OVERFLOW);
     }
 }
 
@@ -572,6 +580,7 @@ __global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* passes, int npa
 // skipped.  Pool slots (one per polygon) and the fan triangles' records
 // (after the setup segments of ws.tris) are reserved once per workgroup and
 // round.
+template <int TS>
 __global__ __launch_bounds__(CLIP_THREADS) void k_clip(const PassDesc* passes, Workspace ws) {
     __shared__ float buf[CLIP_FACES][2][POLY_MAX][CLIP_NF];
     __shared__ uint32_t s_npoly, s_ntri, s_poly0, s_tri0;
@@ -640,7 +649,7 @@ __global__ __launch_bounds__(CLIP_THREADS) void k_clip(const PassDesc* passes, W
             cur ^= 1;
             wave_sync();
         }
-        if (too_large && li == 0) atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_POLY_TOO_LARGE);
+        if (too_large && li == 0) raise_flag(ws, FLAG_POLY_TOO_LARGE);
         float(*poly)[CLIP_NF] = buf[g][cur];
         bool keep = n >= 3;                          // uniform within the group
         // fan triangle (0, li+1, li+2) of the polygon in lane li
@@ -659,13 +668,13 @@ __global__ __launch_bounds__(CLIP_THREADS) void k_clip(const PassDesc* passes, W
         uint32_t pid = __shfl(s_poly0 + lpoly, lane & ~(CLIP_GROUP - 1));
         bool reserved = ok;
         if (keep && pid >= ws.poly_cap) {
-            if (li == 0) atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_POLY_OVERFLOW);
+            if (li == 0) raise_flag(ws, FLAG_POLY_OVERFLOW);
             keep = false;
             ok = false;
         }
         uint32_t slot = s_tri0 + ltri;
         if (reserved && slot >= clip_cap) {
-            atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_TRIS_OVERFLOW);
+            raise_flag(ws, FLAG_TRIS_OVERFLOW);
             ok = reserved = false;
         }
         // a reserved record of a dropped polygon is written empty (no tiles) for k_bin
@@ -683,8 +692,8 @@ __global__ __launch_bounds__(CLIP_THREADS) void k_clip(const PassDesc* passes, W
             }
             P.poly_of_face[f] = keep ? (int)pid : -1;
         }
-        count_tiles<0>(P, ws, pass, ok, t, CLIPPED_BIT | (pid << 3) | (uint32_t)li, ws.seg_region + slot, 0u,
-                       nullptr);
+        count_tiles<0, TS>(P, ws, pass, ok, t, CLIPPED_BIT | (pid << FAN_BITS) | (uint32_t)li, ws.seg_region + slot,
+                           0u, nullptr);
         __syncthreads();
     }
 }
@@ -730,7 +739,7 @@ __device__ __forceinline__ void bin_record(const PassDesc* passes, const Workspa
 template <int BITS>
 __global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* passes, Workspace ws, int nseg) {
     constexpr int BIN_HASH = 1 << BITS;
-    if (ws.status[STATUS_FLAGS] & FLAG_PAIR_LIMIT) return;   // nothing is rasterised
+    if (ws.status[STATUS_SEQ_FLAGS] & FLAG_NO_RASTER) return;   // nothing is rasterised
     if ((int)blockIdx.x < nseg) {
         __shared__ uint32_t keys[BIN_HASH];
         __shared__ uint32_t cnt[BIN_HASH];   // a slot's next bin position, set when its tile is inserted
@@ -909,9 +918,14 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(Workspace ws) {
         uint32_t total = (uint32_t)min(run, (uint64_t)MAX_PAIRS);
         ws.tile_offset[n] = total;
         ws.status[STATUS_BINS_NEEDED] = total;
-        atomicMax(&ws.status[STATUS_NEED_BINS], total);
-        if (run > ws.bin_cap) atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_BIN_OVERFLOW);
-        if (run > MAX_PAIRS) atomicOr(&ws.status[STATUS_FLAGS], (uint32_t)FLAG_PAIR_LIMIT);
+        // past the pair limit no workspace would do: that flag alone, and no need
+        // for the host to grow the bins by
+        if (run > MAX_PAIRS) {
+            raise_flag(ws, FLAG_PAIR_LIMIT);
+        } else {
+            atomicMax(&ws.status[STATUS_NEED_BINS], total);
+            if (run > ws.bin_cap) raise_flag(ws, FLAG_BIN_OVERFLOW);
+        }
         // every status word is final now (binning only reads them)
         if (ws.status_out)
             for (int i = 0; i < STATUS_WORDS; i++)
@@ -925,15 +939,18 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(Workspace ws) {
 
 constexpr int SMALL_PIX = HANA_SMALL_PIX;   // triangles with at most this many candidate pixels: one lane each
 
-// k / w for 0 <= k < TILE_PIX and 1 <= w <= TILE as (k * magic) >> 16 with
-// magic = ceil(2^16 / w): the excess k * (magic - 2^16/w) / 2^16 stays below
-// 1/256, less than the gap 1/w between k/w and the next integer.
+// k / w for 0 <= k < 1024 and 1 <= w <= 32 (a tile's pixels and row widths)
+// as (k * magic) >> 16 with magic = ceil(2^16 / w): the excess
+// k * (magic - 2^16/w) / 2^16 stays below 1024 / 2^16 = 1/64, less than the
+// gap 1/w >= 1/32 between k/w and the next integer.
 __device__ __forceinline__ uint32_t tile_div_magic(int w) { return (65536u + (uint32_t)w - 1u) / (uint32_t)w; }
 __device__ __forceinline__ int tile_div(int k, uint32_t magic) { return (int)(((uint32_t)k * magic) >> 16); }
 
+template <int TS>
 struct TileLds {
-    uint64_t key[TILE_PIX];
-    uint32_t nan_order[TILE_PIX];   // 1 + order of the last NaN-depth fragment
+    static constexpr int PIX = 1 << (2 * TS);
+    uint64_t key[PIX];
+    uint32_t nan_order[PIX];   // 1 + order of the last NaN-depth fragment
     float ax[RBLOCK], ay[RBLOCK], s0x[RBLOCK], s0y[RBLOCK], s1x[RBLOCK], s1y[RBLOCK], uz[RBLOCK];
     float z0[RBLOCK], z1[RBLOCK], z2[RBLOCK];
     uint32_t lo[RBLOCK], box[RBLOCK];
@@ -949,7 +966,7 @@ __device__ __forceinline__ bool load_tri(const PassDesc& P, const Workspace& ws,
         a = P.vr[i0]; b = P.vr[i1]; c = P.vr[i2];
         order = (uint32_t)f << 4;
     } else {
-        uint32_t pid = (e & ~CLIPPED_BIT) >> 3, j = e & 7u;
+        uint32_t pid = (e & ~CLIPPED_BIT) >> FAN_BITS, j = e & ((1u << FAN_BITS) - 1u);
         const float4* pv = ws.pool_vr + (size_t)pid * POLY_MAX;
         a = pv[0]; b = pv[j + 1]; c = pv[j + 2];
         order = ((uint32_t)ws.pool_face[pid] << 4) + 1u + j;
@@ -958,11 +975,12 @@ __device__ __forceinline__ bool load_tri(const PassDesc& P, const Workspace& ws,
 }
 
 // One fragment of triangle t at tile-local pixel (lx, ly).
-__device__ __forceinline__ void frag_resolve(TileLds& L, const TriS& t, uint32_t lo, int xb, int yb, int lx, int ly,
+template <int TS>
+__device__ __forceinline__ void frag_resolve(TileLds<TS>& L, const TriS& t, uint32_t lo, int xb, int yb, int lx, int ly,
                                              bool second) {
     float wx, wy, wz, z;
     if (!tri_cover(t, xb + lx, yb + ly, wx, wy, wz, z)) return;
-    int i = (ly << TILE_SHIFT) | lx;
+    int i = (ly << TS) | lx;
     if (!second) {
         if (z != z) {
             atomicMax(&L.nan_order[i], (0xFFFFFFFEu - lo) + 1u);
@@ -981,8 +999,10 @@ __device__ __forceinline__ void frag_resolve(TileLds& L, const TriS& t, uint32_t
 
 // One sweep over the tile's bin: small triangles one per lane, larger ones
 // one per wave with the lanes spread over their candidate pixels.
-__device__ void sweep(TileLds& L, const PassDesc& P, const Workspace& ws, uint32_t beg, uint32_t cnt, int xb, int yb,
+template <int TS>
+__device__ void sweep(TileLds<TS>& L, const PassDesc& P, const Workspace& ws, uint32_t beg, uint32_t cnt, int xb, int yb,
                       bool second) {
+    constexpr int EDGE = 1 << TS;
     int tid = threadIdx.x, lane = tid & 63;
     bool er = cnt > (uint32_t)HANA_ER_MIN_ENTRIES;   // uniform over the workgroup
     for (uint32_t base = 0; base < cnt; base += RBLOCK) {
@@ -992,8 +1012,8 @@ __device__ void sweep(TileLds& L, const PassDesc& P, const Workspace& ws, uint32
         if (base + tid < cnt) ok = load_tri(P, ws, ws.bins[beg + base + tid], t, order);
         int bx0 = 0, bx1 = -1, by0 = 0, by1 = -1;
         if (ok) {
-            bx0 = max(t.x0, xb) - xb; bx1 = min(t.x1, xb + TILE - 1) - xb;
-            by0 = max(t.y0, yb) - yb; by1 = min(t.y1, yb + TILE - 1) - yb;
+            bx0 = max(t.x0, xb) - xb; bx1 = min(t.x1, xb + EDGE - 1) - xb;
+            by0 = max(t.y0, yb) - yb; by1 = min(t.y1, yb + EDGE - 1) - yb;
             ok = bx1 >= bx0 && by1 >= by0;
         }
         uint32_t lo = 0xFFFFFFFEu - order;
@@ -1013,11 +1033,11 @@ __device__ void sweep(TileLds& L, const PassDesc& P, const Workspace& ws, uint32
                         int b = __builtin_ctz(mask);
                         mask &= mask - 1u;
                         int row = tile_div(b, magic);
-                        frag_resolve(L, t, lo, xb, yb, bx0 + b - row * wdt, by0 + row, second);
+                        frag_resolve<TS>(L, t, lo, xb, yb, bx0 + b - row * wdt, by0 + row, second);
                     }
                 } else
                     for (int ly = by0; ly <= by1; ly++)
-                        for (int lx = bx0; lx <= bx1; lx++) frag_resolve(L, t, lo, xb, yb, lx, ly, second);
+                        for (int lx = bx0; lx <= bx1; lx++) frag_resolve<TS>(L, t, lo, xb, yb, lx, ly, second);
             } else {
                 L.ax[tid] = t.ax; L.ay[tid] = t.ay; L.s0x[tid] = t.s0x; L.s0y[tid] = t.s0y;
                 L.s1x[tid] = t.s1x; L.s1y[tid] = t.s1y; L.uz[tid] = t.uz;
@@ -1046,7 +1066,7 @@ __device__ void sweep(TileLds& L, const PassDesc& P, const Workspace& ws, uint32
             uint32_t magic = tile_div_magic(wdt);
             for (int k = lane; k < npix; k += 64) {
                 int row = tile_div(k, magic);
-                frag_resolve(L, bt, blo, xb, yb, x0 + k - row * wdt, y0 + row, second);
+                frag_resolve<TS>(L, bt, blo, xb, yb, x0 + k - row * wdt, y0 + row, second);
             }
         }
         __syncthreads();
@@ -1094,11 +1114,16 @@ __device__ __forceinline__ void shade_pixel(const PassDesc& P, const Workspace& 
         }
     }
     Tex dt{P.diffuse, P.dw, P.dh}, nt{P.normal, P.nw, P.nh};
-    ShadowMap sm{P.shadow, P.sw, P.sh};
-    C3 col = fragment_program(P.kind, *P.u, vy, dt, nt, sm);
+    ShadowMap sm{P.shadow, P.shadow_depth, P.sw, P.sh};
     size_t pix = (size_t)py * P.width + px;
-    uint32_t alpha = P.clear ? P.clear_rgba : P.color[pix];
-    P.color[pix] = pack_rgb(col, alpha);
+    // the fp32 extension's shadow pass is depth-only
+    if (!(P.kind == SHADER_SHADOW && P.u->shadow_fp32)) {
+        C3 col = fragment_program(P.kind, *P.u, vy, dt, nt, sm);
+        uint32_t alpha = P.clear ? P.clear_rgba : P.color[pix];
+        P.color[pix] = pack_rgb(col, alpha);
+    } else if (P.clear) {
+        P.color[pix] = P.clear_rgba;
+    }
     // A NaN depth is stored as the x86 default NaN (quiet, sign set): that is
     // what the reference's own arithmetic produces for it (an invalid
     // operation such as inf/inf yields that pattern and NaN operands pass it on).
@@ -1124,19 +1149,39 @@ __device__ __forceinline__ int xcd_logical_block(int b, int nb) {
     return ((slot / ch) * kXcd + xcd) * ch + slot % ch;
 }
 
-__global__ __launch_bounds__(RBLOCK) void k_raster(const PassDesc* passes, int npasses, Workspace ws) {
-    __shared__ TileLds L;
-    int blk = xcd_logical_block(blockIdx.x, gridDim.x);
-    const PassDesc& P = passes[find_pass<2>(passes, npasses, blk)];
-    int lt = blk - P.tblock0;
+// Renders tile lt of pass P: the visibility of its pixels in LDS, then one
+// fragment program per covered pixel; a tile no triangle reaches only takes
+// the pass's clear.  The workgroup's threads all call it; it leaves L in use
+// until they have all returned.
+template <int TS>
+__device__ void raster_tile(TileLds<TS>& L, const PassDesc& P, const Workspace& ws, int lt) {
+    constexpr int EDGE = 1 << TS, PIX = 1 << (2 * TS);
     int tx = lt % P.tiles_x, ty = lt / P.tiles_x;
     int gt = P.tile0 + lt;
-    int xb = tx << TILE_SHIFT, yb = ty << TILE_SHIFT;
+    int xb = tx << TS, yb = ty << TS;
     int tid = threadIdx.x;
     int W = P.width, H = P.height;
 
-    for (int i = tid; i < TILE_PIX; i += RBLOCK) {
-        int px = xb + (i & (TILE - 1)), py = yb + (i >> TILE_SHIFT);
+    uint32_t beg = ws.tile_offset[gt];
+    uint32_t cnt = ws.tile_count[gt];
+    if (beg >= ws.bin_cap) cnt = 0;
+    else if (cnt > ws.bin_cap - beg) cnt = ws.bin_cap - beg;
+    if (cnt == 0) {   // uniform: no fragment, the clear only
+        for (int i = tid; i < PIX; i += RBLOCK) {
+            int px = xb + (i & (EDGE - 1)), py = yb + (i >> TS);
+            if (px >= W || py >= H) continue;
+            size_t pix = (size_t)py * W + px;
+            if (P.clear) {
+                P.color[pix] = P.clear_rgba;
+                P.depth[pix] = P.clear_depth;
+            }
+            if (P.winner) P.winner[pix] = NO_WINNER;
+        }
+        return;
+    }
+
+    for (int i = tid; i < PIX; i += RBLOCK) {
+        int px = xb + (i & (EDGE - 1)), py = yb + (i >> TS);
         uint64_t k = ~0ull;
         if (px < W && py < H) k = init_key(P.clear ? P.clear_depth : P.depth[(size_t)py * W + px]);
         L.key[i] = k;
@@ -1145,26 +1190,21 @@ __global__ __launch_bounds__(RBLOCK) void k_raster(const PassDesc* passes, int n
     if (tid == 0) { L.nbig = 0; L.head = 0; L.nan_seen = 0; }
     __syncthreads();
 
-    uint32_t beg = ws.tile_offset[gt];
-    uint32_t cnt = ws.tile_count[gt];
-    // past MAX_PAIRS the bins are not written and their positions may wrap
-    if (beg >= ws.bin_cap || (ws.status[STATUS_FLAGS] & FLAG_PAIR_LIMIT)) cnt = 0;
-    else if (cnt > ws.bin_cap - beg) cnt = ws.bin_cap - beg;
-    sweep(L, P, ws, beg, cnt, xb, yb, false);
+    sweep<TS>(L, P, ws, beg, cnt, xb, yb, false);
 
     if (L.nan_seen) {
         // pixels that saw a NaN-depth fragment restart after the last one
-        for (int i = tid; i < TILE_PIX; i += RBLOCK) {
+        for (int i = tid; i < PIX; i += RBLOCK) {
             uint32_t no = L.nan_order[i];
             if (no) L.key[i] = (0xFFFFFFFFull << 32) | (uint64_t)(0xFFFFFFFEu - (no - 1u));
         }
         __syncthreads();
-        sweep(L, P, ws, beg, cnt, xb, yb, true);
+        sweep<TS>(L, P, ws, beg, cnt, xb, yb, true);
     }
 
     uint32_t written = 0;
-    for (int i = tid; i < TILE_PIX; i += RBLOCK) {
-        int px = xb + (i & (TILE - 1)), py = yb + (i >> TILE_SHIFT);
+    for (int i = tid; i < PIX; i += RBLOCK) {
+        int px = xb + (i & (EDGE - 1)), py = yb + (i >> TS);
         if (px >= W || py >= H) continue;
         uint32_t lo = (uint32_t)L.key[i];
         size_t pix = (size_t)py * W + px;
@@ -1180,6 +1220,98 @@ __global__ __launch_bounds__(RBLOCK) void k_raster(const PassDesc* passes, int n
         written++;
     }
     if (P.stats) stat_add(P, ST_PIXELS, written);
+}
+
+// One workgroup per tile of one dependency group (passes of the group, their
+// tiles numbered from 0 by tblock0), XCD-banded.  GROUP only names the launch
+// (the shadow passes' group 0, the main passes' group 1) in profiles.
+template <int TS, int GROUP>
+__global__ __launch_bounds__(RBLOCK) void k_raster(const PassDesc* passes, int npasses, Workspace ws) {
+    __shared__ TileLds<TS> L;
+    // an incomplete sequence leaves its targets untouched: the host renders it again
+    if (ws.status[STATUS_SEQ_FLAGS] & FLAG_NO_RASTER) return;
+    int blk = xcd_logical_block(blockIdx.x, gridDim.x);
+    const PassDesc& P = passes[find_pass<2>(passes, npasses, blk)];
+    raster_tile<TS>(L, P, ws, blk - P.tblock0);
+}
+
+// The persistent raster: a fixed grid of workgroups takes the tiles of every
+// pass of the sequence from one queue, RQ_CHUNK consecutive tiles at a time
+// (they share triangles, which stay in the taking XCD's L2).  Queue items are
+// global tile indices: the shadow passes' tiles first, then the main passes
+// that sample no shadow map drawn here, then those that do.  A main pass of
+// the last kind needs every shadow tile finished:
+//  - a workgroup counts the shadow tiles it finishes and publishes the count
+//    once, with one release at agent scope (its shadow-map writes become
+//    visible on every XCD), when it first takes an item past them or finds
+//    the queue empty;
+//  - before its first item that samples a map, it waits until the published
+//    total is every shadow tile, then acquires once.
+// Items are taken in queue order, so every shadow tile was taken before any
+// item that waits, by a workgroup that publishes before it could wait itself:
+// the wait ends.  It is still bounded: past RQ_SPIN_LIMIT polls it raises
+// FLAG_QUEUE_STALL (an error for the host) and goes on.
+constexpr uint32_t RQ_CHUNK = 4;
+constexpr uint32_t RQ_SPIN_LIMIT = 1u << 24;
+
+// largest pass index whose first global tile is <= item (passes sorted by it)
+__device__ __forceinline__ int pass_of_item(const PassDesc* P, int n, uint32_t item) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if ((uint32_t)P[mid].tile0 <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int TS>
+__global__ __launch_bounds__(RBLOCK) void k_raster_q(const PassDesc* passes, int npasses, Workspace ws) {
+    __shared__ TileLds<TS> L;
+    __shared__ uint32_t s_first;
+    if (ws.status[STATUS_SEQ_FLAGS] & FLAG_NO_RASTER) return;
+    const int tid = threadIdx.x;
+    uint32_t mine = 0;   // shadow tiles finished here, not yet published
+    bool published = false, ready = false;
+    for (;;) {
+        if (tid == 0) s_first = atomicAdd(&ws.qstate[0], RQ_CHUNK);
+        __syncthreads();
+        uint32_t first = s_first;
+        __syncthreads();
+        uint32_t end = min(first + RQ_CHUNK, ws.q_items);
+        if (!published && (first >= ws.q_g0 || first >= ws.q_items)) {
+            if (tid == 0 && mine)
+                __hip_atomic_fetch_add(&ws.qstate[1], mine, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            published = true;
+        }
+        if (first >= ws.q_items) break;
+        for (uint32_t it = first; it < end; it++) {
+            if (!published && it >= ws.q_g0) {
+                if (tid == 0 && mine)
+                    __hip_atomic_fetch_add(&ws.qstate[1], mine, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                published = true;
+            }
+            if (!ready && it >= ws.q_dep_from) {
+                if (tid == 0) {
+                    uint32_t spins = 0;
+                    while (__hip_atomic_load(&ws.qstate[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < ws.q_g0) {
+                        if (++spins > RQ_SPIN_LIMIT) {
+                            raise_flag(ws, FLAG_QUEUE_STALL);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(8);
+                    }
+                }
+                __syncthreads();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                ready = true;
+            }
+            const PassDesc& P = passes[pass_of_item(passes, npasses, it)];
+            raster_tile<TS>(L, P, ws, (int)(it - (uint32_t)P.tile0));
+            __syncthreads();   // the tile's writes are issued and L is free again
+            if (it < ws.q_g0) mine++;
+        }
+    }
 }
 
 // fills a 32-bit buffer (graph-safe clear)
@@ -1217,15 +1349,19 @@ hipError_t launch_vertex(const PassDesc* d_passes, int npasses, const Workspace&
 hipError_t launch_setup(const PassDesc* d_passes, int npasses, const Workspace& ws, int blocks, bool small_tris,
                         hipStream_t s) {
     if (blocks <= 0) return hipSuccess;
-    if (small_tris)
-        hipLaunchKernelGGL(k_setup<9>, dim3(blocks), dim3(BLOCK), 0, s, d_passes, npasses, ws);
-    else
-        hipLaunchKernelGGL(k_setup<10>, dim3(blocks), dim3(BLOCK), 0, s, d_passes, npasses, ws);
+    if (ws.tile_shift == 5) {
+        if (small_tris) hipLaunchKernelGGL((k_setup<9, 5>), dim3(blocks), dim3(BLOCK), 0, s, d_passes, npasses, ws);
+        else hipLaunchKernelGGL((k_setup<10, 5>), dim3(blocks), dim3(BLOCK), 0, s, d_passes, npasses, ws);
+    } else {
+        if (small_tris) hipLaunchKernelGGL((k_setup<9, 4>), dim3(blocks), dim3(BLOCK), 0, s, d_passes, npasses, ws);
+        else hipLaunchKernelGGL((k_setup<10, 4>), dim3(blocks), dim3(BLOCK), 0, s, d_passes, npasses, ws);
+    }
     return hipGetLastError();
 }
 hipError_t launch_clip(const PassDesc* d_passes, const Workspace& ws, hipStream_t s) {
     // grid-stride: the queue length is only known on the device
-    hipLaunchKernelGGL(k_clip, dim3(1024), dim3(CLIP_THREADS), 0, s, d_passes, ws);
+    if (ws.tile_shift == 5) hipLaunchKernelGGL(k_clip<5>, dim3(2048), dim3(CLIP_THREADS), 0, s, d_passes, ws);
+    else hipLaunchKernelGGL(k_clip<4>, dim3(2048), dim3(CLIP_THREADS), 0, s, d_passes, ws);
     return hipGetLastError();
 }
 int scan_chunks(int ntiles) { return ntiles > 0 ? (ntiles + SCAN_CHUNK - 1) / SCAN_CHUNK : 1; }
@@ -1248,9 +1384,25 @@ hipError_t launch_bin(const PassDesc* d_passes, const Workspace& ws, int nseg, b
         hipLaunchKernelGGL(k_bin<11>, dim3(nseg + 512), dim3(BLOCK), 0, s, d_passes, ws, nseg);
     return hipGetLastError();
 }
-hipError_t launch_raster(const PassDesc* d_passes, int npasses, const Workspace& ws, int blocks, hipStream_t s) {
+hipError_t launch_raster(const PassDesc* d_passes, int npasses, const Workspace& ws, int blocks, int group,
+                         hipStream_t s) {
     if (blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_raster, dim3(blocks), dim3(RBLOCK), 0, s, d_passes, npasses, ws);
+    if (ws.tile_shift == 5) {
+        if (group) hipLaunchKernelGGL((k_raster<5, 1>), dim3(blocks), dim3(RBLOCK), 0, s, d_passes, npasses, ws);
+        else hipLaunchKernelGGL((k_raster<5, 0>), dim3(blocks), dim3(RBLOCK), 0, s, d_passes, npasses, ws);
+    } else {
+        if (group) hipLaunchKernelGGL((k_raster<4, 1>), dim3(blocks), dim3(RBLOCK), 0, s, d_passes, npasses, ws);
+        else hipLaunchKernelGGL((k_raster<4, 0>), dim3(blocks), dim3(RBLOCK), 0, s, d_passes, npasses, ws);
+    }
+    return hipGetLastError();
+}
+hipError_t launch_raster_queue(const PassDesc* d_passes, int npasses, const Workspace& ws, int workgroups,
+                               hipStream_t s) {
+    if (workgroups <= 0 || ws.q_items == 0) return hipSuccess;
+    if (ws.tile_shift == 5)
+        hipLaunchKernelGGL(k_raster_q<5>, dim3(workgroups), dim3(RBLOCK), 0, s, d_passes, npasses, ws);
+    else
+        hipLaunchKernelGGL(k_raster_q<4>, dim3(workgroups), dim3(RBLOCK), 0, s, d_passes, npasses, ws);
     return hipGetLastError();
 }
 hipError_t launch_present(const uint32_t* src, uint32_t* dst, int w, int h, hipStream_t s) {

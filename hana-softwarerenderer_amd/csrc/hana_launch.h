@@ -14,7 +14,12 @@ int scan_chunks(int ntiles);
 hipError_t launch_scan(const Workspace& ws, hipStream_t s);
 // small_tris: every pass averages at most 4 pixels per face (smaller tile table)
 hipError_t launch_bin(const PassDesc* d_passes, const Workspace& ws, int nseg, bool small_tris, hipStream_t s);
-hipError_t launch_raster(const PassDesc* d_passes, int npasses, const Workspace& ws, int blocks, hipStream_t s);
+// one workgroup per tile of dependency group `group` (ws.tile_shift selects the tile size)
+hipError_t launch_raster(const PassDesc* d_passes, int npasses, const Workspace& ws, int blocks, int group,
+                         hipStream_t s);
+// the persistent raster over every pass of the sequence: `workgroups` workgroups take tiles from a queue
+hipError_t launch_raster_queue(const PassDesc* d_passes, int npasses, const Workspace& ws, int workgroups,
+                               hipStream_t s);
 hipError_t launch_fill32(uint32_t* dst, uint32_t value, size_t n, hipStream_t s);
 hipError_t launch_present(const uint32_t* src, uint32_t* dst, int w, int h, hipStream_t s);
 

@@ -119,11 +119,35 @@ int materialize_clear(hana_ctx* c, hana_target* t, hipStream_t s) {
     return HANA_OK;
 }
 
-// Lays out the workspace of a set of passes.  Passes are ordered by raster
-// group (stable); every per-pass slice is 256-vertex aligned.
-int build_plan(hana_ctx* c, std::vector<PassSpec> specs, Plan& pl) {
-    std::stable_sort(specs.begin(), specs.end(), [](const PassSpec& a, const PassSpec& b) { return a.group < b.group; });
-    pl = Plan();
+// Lays out the workspace of a set of passes.  Passes are ordered (stably) by
+// raster group, and within the main group those that sample a shadow map
+// drawn by the first group come last: the persistent raster's queue holds the
+// tiles in pass order, so every shadow tile precedes the tiles that wait for
+// them.  Every per-pass slice is 1024-vertex aligned.
+int build_plan(hana_ctx* c, std::vector<PassSpec> specs, Plan& pl, const Tuning& tu) {
+    {
+        std::vector<std::pair<int, size_t>> key(specs.size());
+        for (size_t i = 0; i < specs.size(); i++) {
+            const PassSpec& s = specs[i];
+            bool dep = false;
+            if (s.shadow && s.group == 1)
+                for (const PassSpec& o : specs) dep = dep || (o.group == 0 && o.target == s.shadow);
+            key[i] = {s.group * 2 + (dep ? 1 : 0), i};
+        }
+        std::stable_sort(key.begin(), key.end(),
+                         [](const std::pair<int, size_t>& a, const std::pair<int, size_t>& b) { return a.first < b.first; });
+        std::vector<PassSpec> sorted;
+        sorted.reserve(specs.size());
+        for (auto& k : key) sorted.push_back(specs[k.second]);
+        pl = Plan();
+        specs.swap(sorted);
+        // the first pass that waits for the shadow tiles
+        size_t first_dep = specs.size();
+        for (size_t i = 0; i < key.size(); i++)
+            if (key[i].first == 3) { first_dep = i; break; }
+        pl.q_dep_from = (uint32_t)first_dep;   // a pass index until the tiles are known (below)
+    }
+    const int tshift = tu.tile_shift, tedge = tile_edge(tu.tile_shift);
     pl.specs = specs;
     int n = (int)specs.size();
     pl.descs.resize(n);
@@ -151,12 +175,17 @@ int build_plan(hana_ctx* c, std::vector<PassSpec> specs, Plan& pl) {
         d.idx = m->idx;
         if (s.diffuse) { d.diffuse = s.diffuse->texels; d.dw = s.diffuse->w; d.dh = s.diffuse->h; }
         if (s.normal) { d.normal = s.normal->texels; d.nw = s.normal->w; d.nh = s.normal->h; }
-        if (s.shadow) { d.shadow = s.shadow->color; d.sw = s.shadow->w; d.sh = s.shadow->h; }
+        if (s.shadow) {
+            d.shadow = s.shadow->color;
+            d.shadow_depth = s.shadow->depth;
+            d.sw = s.shadow->w;
+            d.sh = s.shadow->h;
+        }
         hana_target* t = s.target;
         d.color = t->color; d.depth = t->depth; d.winner = t->winner;
         d.width = t->w; d.height = t->h;
-        d.tiles_x = (t->w + TILE - 1) / TILE;
-        d.tiles_y = (t->h + TILE - 1) / TILE;
+        d.tiles_x = (t->w + tedge - 1) / tedge;
+        d.tiles_y = (t->h + tedge - 1) / tedge;
         if ((int64_t)t->w * t->h > 4 * (int64_t)m->F) pl.small_tris = false;
         const size_t vchunk = (size_t)BLOCK * VPT, fchunk = (size_t)BLOCK * FPT;
         size_t vpad = (V + vchunk - 1) / vchunk * vchunk;
@@ -182,6 +211,10 @@ int build_plan(hana_ctx* c, std::vector<PassSpec> specs, Plan& pl) {
     pl.total_vs = vsoff;
     pl.total_f = fo;
     pl.total_tiles = tile0;
+    pl.tile_shift = tshift;
+    pl.raster_wgs = tu.raster_wgs_per_cu > 0 ? tu.raster_wgs_per_cu * c->num_cus : 0;
+    pl.q_g0 = (uint32_t)tb[0];
+    pl.q_dep_from = pl.q_dep_from < (uint32_t)n ? (uint32_t)pl.descs[pl.q_dep_from].tile0 : (uint32_t)tile0;
     pl.ngroups = 0;
     for (int g = 0; g < 2; g++) {
         int first = -1, count = 0;
@@ -222,7 +255,7 @@ int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_p
         A.bin_cap = (uint32_t)want_bins;
     }
     uint64_t want_polys = std::max<uint64_t>({(uint64_t)min_polys, pl.total_f / 16 + 4096});
-    want_polys = std::min<uint64_t>(want_polys, (1u << 28) - 1);
+    want_polys = std::min<uint64_t>(want_polys, POOL_CAP_MAX);
     if (want_polys > A.poly_cap) {
         if ((r = ensure(c, A.pool_vr, sizeof(float4) * POLY_MAX * want_polys))) return r;
         if ((r = ensure(c, A.pool_vs, sizeof(float) * POLY_MAX * POLY_VARY * want_polys))) return r;
@@ -247,6 +280,7 @@ int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_p
     if ((r = ensure(c, A.seg_tiles, sizeof(uint2) * seg_stride * nsegs))) return r;
     if ((r = ensure(c, A.seg_wide, sizeof(uint16_t) * BLOCK * FPT * nsegs))) return r;
     if ((r = ensure(c, A.seg_info, sizeof(uint32_t) * nsegs))) return r;
+    if ((r = ensure(c, A.qstate, 256))) return r;
     // every face may fail the trivial accept (camera inside the mesh)
     uint64_t want_q = std::max<uint64_t>(pl.total_f, 1024);
     want_q = std::min<uint64_t>(want_q, 0xFFFFFFF0ull);
@@ -294,6 +328,11 @@ int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_p
     ws.ntiles = pl.total_tiles;
     ws.bin_cap = A.bin_cap;
     ws.poly_cap = A.poly_cap;
+    ws.qstate = A.qstate.as<uint32_t>();
+    ws.q_items = (uint32_t)pl.total_tiles;
+    ws.q_g0 = pl.q_g0;
+    ws.q_dep_from = pl.q_dep_from;
+    ws.tile_shift = pl.tile_shift;
     return HANA_OK;
 }
 
@@ -374,13 +413,20 @@ int enqueue_geometry(hana_ctx* c, const Plan& pl, const Arena& A, hipStream_t s,
     return HANA_OK;
 }
 
-// The raster stages, one launch per dependency group.
+// The raster stages: one launch per dependency group, or one persistent
+// launch for both (timed as the first group's stage).
 int enqueue_raster(hana_ctx* c, const Plan& pl, const Arena& A, hipStream_t s, bool timed) {
     const PassDesc* d = A.tables.as<PassDesc>();
+    if (pl.raster_wgs > 0) {
+        StageTimer t(c, 6, timed, s);
+        HIP_TRY(c, launch_raster_queue(d, (int)pl.descs.size(), pl.ws, pl.raster_wgs, s));
+        t.done(s);
+        return HANA_OK;
+    }
     for (int g = 0; g < 2; g++)
         if (pl.group_count[g]) {
             StageTimer t(c, 6 + g, timed, s);
-            HIP_TRY(c, launch_raster(d + pl.group_first[g], pl.group_count[g], pl.ws, pl.group_blocks[g], s));
+            HIP_TRY(c, launch_raster(d + pl.group_first[g], pl.group_count[g], pl.ws, pl.group_blocks[g], g, s));
             t.done(s);
         }
     return HANA_OK;
@@ -440,7 +486,7 @@ static int grow_arena(hana_ctx* c, Arena& A, uint32_t flags, uint32_t need_bins,
     }
     if (flags & FLAG_POLY_OVERFLOW) {
         uint64_t want = std::max<uint64_t>((uint64_t)need_polys + need_polys / 4 + 1024, A.poly_cap);
-        want = std::min<uint64_t>(want, (1u << 28) - 1);
+        want = std::min<uint64_t>(want, POOL_CAP_MAX);
         int r;
         if ((r = ensure(c, A.pool_vr, sizeof(float4) * POLY_MAX * want))) return r;
         if ((r = ensure(c, A.pool_vs, sizeof(float) * POLY_MAX * POLY_VARY * want))) return r;
@@ -452,47 +498,106 @@ static int grow_arena(hana_ctx* c, Arena& A, uint32_t flags, uint32_t need_bins,
     return HANA_OK;
 }
 
+// Renders a logged call again on the context stream with slot 0's workspace
+// (every stream is idle: the status check has waited for them).
+static int replay_call(hana_ctx* c, LoggedCall& L) {
+    int r;
+    Plan& pl = L.pl;
+    Arena& A = slot_arena(c, 0);
+    if ((r = bind_plan(c, pl, A, 0, 0, 0, false, c->stream))) return r;
+    pl.ws.status_out = c->status_dev[0];
+    if ((r = upload_tables(c, pl, A, c->stream))) return r;
+    if ((r = enqueue_geometry(c, pl, A, c->stream, false))) return r;
+    for (const Fill& f : L.fills) {
+        size_t n = (size_t)f.target->w * f.target->h;
+        HIP_TRY(c, launch_fill32(f.target->color, f.rgba, n, c->stream));
+        uint32_t db;
+        std::memcpy(&db, &f.depth, 4);
+        HIP_TRY(c, launch_fill32(reinterpret_cast<uint32_t*>(f.target->depth), db, n, c->stream));
+    }
+    return enqueue_raster(c, pl, A, c->stream, false);
+}
+
 // Reads the sticky status of every sequence since the last check, over all
-// draw slots (waits for every stream).  On an overflow both workspaces are
-// grown to the largest need seen, the sticky words are cleared, and
-// HANA_E_OVERFLOW asks the caller to render again.
+// draw slots (waits for every stream).
+// - A capacity overflow (bins, clip pool, clip queue, triangle list) left its
+//   sequence's targets untouched.  Every slot's workspace is grown to the
+//   largest need seen and the calls since the last check are rendered again
+//   in call order; a draw over the same state twice leaves it as drawn once
+//   (the depth test keeps the last of the fragments at the minimum depth), so
+//   the targets end as the calls in order leave them.  With replay_overflow
+//   off, the caller is asked to render again (HANA_E_OVERFLOW).
+// - More (triangle, tile) pairs than 32-bit bins reach: that sequence was not
+//   rasterised (HANA_E_NOMEM).
+// - A clipped polygon with more than POLY_MAX vertices was dropped, the rest
+//   drawn (HANA_E_UNSUPPORTED).
 int check_status(hana_ctx* c) {
     if (!c->status_pending) return HANA_OK;
-    int r = sync_all(c);
-    if (r) return r;
-    c->status_pending = false;
-    uint32_t flags = 0, need_bins = 0, need_polys = 0, need_tris = 0;
-    for (int k = 0; k < c->nslots; k++) {
-        const uint32_t* host = slot_status(c, k);
-        if (!host) continue;
-        flags |= host[STATUS_FLAGS];
-        need_bins = std::max(need_bins, host[STATUS_NEED_BINS]);
-        need_polys = std::max(need_polys, host[STATUS_NEED_POLYS]);
-        need_tris = std::max(need_tris, host[STATUS_NEED_TRIS]);
+    int r;
+    uint32_t seen = 0;
+    for (int attempt = 0;; attempt++) {
+        if ((r = sync_all(c))) return r;
+        uint32_t flags = 0, need_bins = 0, need_polys = 0, need_tris = 0;
+        for (int k = 0; k < c->nslots; k++) {
+            const uint32_t* host = slot_status(c, k);
+            if (!host) continue;
+            flags |= host[STATUS_FLAGS];
+            need_bins = std::max(need_bins, host[STATUS_NEED_BINS]);
+            need_polys = std::max(need_polys, host[STATUS_NEED_POLYS]);
+            need_tris = std::max(need_tris, host[STATUS_NEED_TRIS]);
+        }
+        seen |= flags;
+        if (flags) {
+            for (int k = 0; k < c->nslots; k++) {
+                uint32_t* host = slot_status(c, k);
+                if (!host) continue;
+                Arena& A = slot_arena(c, k);
+                if (A.status.p)
+                    HIP_TRY(c, launch_fill32(A.status.as<uint32_t>() + STATUS_SEQ_WORDS, 0u,
+                                             STATUS_WORDS - STATUS_SEQ_WORDS, c->stream));
+                std::memset(host + STATUS_SEQ_WORDS, 0, sizeof(uint32_t) * (STATUS_WORDS - STATUS_SEQ_WORDS));
+            }
+            // the next geometry stages may run on any stream
+            HIP_TRY(c, hipStreamSynchronize(c->stream));
+        }
+        uint32_t cap = flags & FLAG_CAPACITY;
+        if (cap)
+            for (int k = 0; k < c->nslots; k++)
+                if ((r = grow_arena(c, slot_arena(c, k), flags, need_bins, need_polys, need_tris))) {
+                    c->log.clear();
+                    c->status_pending = false;
+                    return r;
+                }
+        if (cap && c->replay_overflow && attempt < 3) {
+            c->replays++;
+            for (LoggedCall& L : c->log)
+                if ((r = replay_call(c, L))) {
+                    c->log.clear();
+                    c->status_pending = false;
+                    return r;
+                }
+            continue;   // wait for the replay and check it
+        }
+        c->log.clear();
+        c->status_pending = false;
+        if (cap)
+            return fail(c, HANA_E_OVERFLOW,
+                        c->replay_overflow ? "workspace capacity still exceeded after growing it three times"
+                                           : "bin or clip-pool capacity exceeded since the last sync; buffers were "
+                                             "grown, render those frames again");
+        break;
     }
-    if (!flags) return HANA_OK;
-    for (int k = 0; k < c->nslots; k++) {
-        uint32_t* host = slot_status(c, k);
-        if (!host) continue;
-        Arena& A = slot_arena(c, k);
-        if (A.status.p)
-            HIP_TRY(c, launch_fill32(A.status.as<uint32_t>() + STATUS_SEQ_WORDS, 0u, STATUS_WORDS - STATUS_SEQ_WORDS,
-                                     c->stream));
-        std::memset(host + STATUS_SEQ_WORDS, 0, sizeof(uint32_t) * (STATUS_WORDS - STATUS_SEQ_WORDS));
-    }
-    // the next geometry stages may run on either stream
-    HIP_TRY(c, hipStreamSynchronize(c->stream));
-    if (flags & FLAG_POLY_TOO_LARGE)
-        return fail(c, HANA_E_OVERFLOW, "a clipped polygon exceeded 10 vertices and was dropped");
-    if (flags & FLAG_PAIR_LIMIT)
+    if (seen & FLAG_QUEUE_STALL)
+        return fail(c, HANA_E_DEVICE, "the persistent raster's wait for the shadow tiles timed out");
+    if (seen & FLAG_PAIR_LIMIT)
         return fail(c, HANA_E_NOMEM,
                     "a draw, frame or batch part covered more than 4,294,967,280 (triangle, tile) pairs and was not "
                     "rasterised; split it");
-    // grow for the next call; the frames since the last check must be re-rendered
-    for (int k = 0; k < c->nslots; k++)
-        if ((r = grow_arena(c, slot_arena(c, k), flags, need_bins, need_polys, need_tris))) return r;
-    return fail(c, HANA_E_OVERFLOW,
-                "bin or clip-pool capacity exceeded since the last sync; buffers were grown, re-render the frame");
+    if (seen & FLAG_POLY_TOO_LARGE)
+        return fail(c, HANA_E_UNSUPPORTED,
+                    "a clipped polygon had more than " + std::to_string(POLY_MAX) +
+                        " vertices and was dropped; the rest was drawn");
+    return HANA_OK;
 }
 
 // Renders a list of passes (per-call path).  Consecutive calls rotate over
@@ -502,16 +607,17 @@ int check_status(hana_ctx* c) {
 // in order.  With timing or statistics on, every call uses slot 0.
 int run_passes(hana_ctx* c, std::vector<PassSpec> specs) {
     int r;
+    if (c->log.size() >= kMaxLoggedCalls && (r = check_status(c))) return r;
     const std::vector<hana_target*> touched = touched_targets(specs);
     std::vector<hana_target*> sampled;   // sampled as a shadow map, not drawn here
     for (auto& s : specs) {
         if (!s.shadow || !s.shadow->pending_clear) continue;
         bool drawn = false;
         for (auto& o : specs) drawn = drawn || o.target == s.shadow;
-        if (!drawn) sampled.push_back(s.shadow);
+        if (!drawn && std::find(sampled.begin(), sampled.end(), s.shadow) == sampled.end()) sampled.push_back(s.shadow);
     }
     Plan pl;
-    if ((r = build_plan(c, std::move(specs), pl))) return r;
+    if ((r = build_plan(c, std::move(specs), pl, c->tuning))) return r;
     bool rotate = c->pipelined && !c->timing && !c->stats_enabled;
     // Large sequences rotate over two slots: with a third in flight, the 2.6 M
     // faces of a 1.3 M-triangle frame with its shadow pass ran 1.2 % slower,
@@ -543,8 +649,11 @@ int run_passes(hana_ctx* c, std::vector<PassSpec> specs) {
             HIP_TRY(c, hipStreamWaitEvent(S, c->slot_done[k], 0));
             c->pend[k][a].clear();
         }
-    for (hana_target* t : sampled)
+    std::vector<Fill> fills;
+    for (hana_target* t : sampled) {
+        fills.push_back({t, t->clear_rgba, t->clear_depth});
         if ((r = materialize_clear(c, t, S))) return r;
+    }
     if ((r = enqueue_raster(c, pl, A, S, true))) return r;
     HIP_TRY(c, hipEventRecord(c->slot_done[a], S));
     if (c->stats_enabled) {
@@ -562,10 +671,21 @@ int run_passes(hana_ctx* c, std::vector<PassSpec> specs) {
     if (c->pipelined) note_pending(c, a, touched);   // else no other slot ever draws
     c->next_slot = (a + 1) % nslots;
     c->status_pending = true;
+    c->log.push_back({std::move(pl), std::move(fills)});
     return HANA_OK;
 }
 
 bool valid_kind(int k) { return k >= HANA_SHADER_GROUND && k <= HANA_SHADER_SHADOW; }
+
+int tuning_from(hana_ctx* c, const hana_tuning* t, Tuning& out) {
+    if (!t || !(t->tile_size == 16 || t->tile_size == 32) || t->raster_wgs_per_cu < 0 || t->raster_wgs_per_cu > 16 ||
+        t->parts < 1 || t->parts > (int)kMaxBatchParts)
+        return fail(c, HANA_E_INVALID, "tuning: tile_size 16 or 32, raster_wgs_per_cu 0-16, parts 1-4");
+    out.tile_shift = t->tile_size == 32 ? 5 : 4;
+    out.raster_wgs_per_cu = t->raster_wgs_per_cu;
+    out.parts = t->parts;
+    return HANA_OK;
+}
 
 }  // namespace rt
 }  // namespace hana
@@ -592,6 +712,9 @@ int create_context(int device, int nslots, hana_ctx** out) {
     hana_ctx* c = new hana_ctx();
     c->device = device;
     c->nslots = nslots;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        c->num_cus = cus;
     auto undo = [c]() {
         for (int i = 0; i < kRing; i++)
             if (c->ring_event[i]) (void)hipEventDestroy(c->ring_event[i]);
@@ -626,15 +749,17 @@ int create_context(int device, int nslots, hana_ctx** out) {
 extern "C" {
 
 static void free_arena(Arena& a) {
-    DevBuf* bufs[] = {&a.tables, &a.vr, &a.vs, &a.code, &a.tris, &a.seg_count, &a.seg_tiles, &a.seg_wide, &a.seg_info, &a.scan_part, &a.pof, &a.tile_count, &a.tile_offset,
-                      &a.tile_cursor, &a.bins, &a.pool_vr, &a.pool_vs, &a.pool_face, &a.pool_pass, &a.pool_n,
-                      &a.clip_queue, &a.status, &a.stats};
+    DevBuf* bufs[] = {&a.tables, &a.vr, &a.vs, &a.code, &a.tris, &a.seg_count, &a.seg_tiles, &a.seg_wide,
+                      &a.seg_info, &a.qstate, &a.scan_part, &a.pof, &a.tile_count, &a.tile_offset, &a.tile_cursor,
+                      &a.bins, &a.pool_vr, &a.pool_vs, &a.pool_face, &a.pool_pass, &a.pool_n, &a.clip_queue,
+                      &a.status, &a.stats};
     for (DevBuf* b : bufs)
         if (b->p) {
             (void)hipFree(b->p);
             b->p = nullptr;
             b->cap = 0;
         }
+    a.bin_cap = a.poly_cap = a.clipq_cap = a.tris_cap = 0;
 }
 
 int hana_gpu_destroy(hana_ctx* c) {
@@ -695,6 +820,12 @@ int hana_gpu_read_timing(hana_ctx* c, double* ms, int64_t* launches) {
         c->ev_pool.push_back(p.second.second);
     }
     c->ev_pending.clear();
+    return HANA_OK;
+}
+
+int hana_gpu_replay_count(hana_ctx* c, int64_t* out) {
+    if (!c || !out) return HANA_E_INVALID;
+    *out = c->replays;
     return HANA_OK;
 }
 
@@ -759,13 +890,16 @@ int hana_mesh_create(hana_ctx* c, int32_t nverts, const float* pos, const float*
     return HANA_OK;
 }
 
+// Objects a logged call may be rendered again with are freed only after the
+// status check (which may replay it); its result is returned.
 int hana_mesh_destroy(hana_ctx* c, hana_mesh* m) {
     if (!c || !m) return HANA_E_INVALID;
+    int st = check_status(c);
     (void)sync_all(c);
     (void)hipFree(m->soa);
     (void)hipFree(m->idx);
     delete m;
-    return HANA_OK;
+    return st;
 }
 
 int hana_texture_create(hana_ctx* c, int32_t w, int32_t h, int32_t bpp, const uint8_t* bytes, hana_texture** out) {
@@ -798,10 +932,11 @@ int hana_texture_create(hana_ctx* c, int32_t w, int32_t h, int32_t bpp, const ui
 
 int hana_texture_destroy(hana_ctx* c, hana_texture* t) {
     if (!c || !t) return HANA_E_INVALID;
+    int st = check_status(c);
     (void)sync_all(c);
     (void)hipFree(t->texels);
     delete t;
-    return HANA_OK;
+    return st;
 }
 
 int hana_target_create(hana_ctx* c, int32_t w, int32_t h, hana_target** out) {
@@ -829,12 +964,13 @@ int hana_target_create(hana_ctx* c, int32_t w, int32_t h, hana_target** out) {
 
 int hana_target_destroy(hana_ctx* c, hana_target* t) {
     if (!c || !t) return HANA_E_INVALID;
+    int st = check_status(c);
     (void)sync_all(c);
     (void)hipFree(t->color);
     (void)hipFree(t->depth);
     if (t->winner) (void)hipFree(t->winner);
     delete t;
-    return HANA_OK;
+    return st;
 }
 
 int hana_target_clear(hana_ctx* c, hana_target* t, const uint8_t rgba[4], float depth) {
@@ -848,7 +984,10 @@ int hana_target_clear(hana_ctx* c, hana_target* t, const uint8_t rgba[4], float 
 int hana_target_upload(hana_ctx* c, hana_target* t, const uint8_t* rgba, const float* depth) {
     if (!c || !t) return HANA_E_INVALID;
     size_t n = (size_t)t->w * t->h;
-    int r = join(c);
+    // draws before the upload are complete (replayed if need be) before it lands
+    int r = check_status(c);
+    if (r) return r;
+    r = join(c);
     if (r || (r = materialize_clear(c, t, c->stream))) return r;
     if (rgba) HIP_TRY(c, hipMemcpyAsync(t->color, rgba, 4 * n, hipMemcpyHostToDevice, c->stream));
     if (depth) HIP_TRY(c, hipMemcpyAsync(t->depth, depth, 4 * n, hipMemcpyHostToDevice, c->stream));
@@ -856,19 +995,25 @@ int hana_target_upload(hana_ctx* c, hana_target* t, const uint8_t* rgba, const f
     return HANA_OK;
 }
 
+// The reads first complete every draw (and render again those that outgrew
+// the workspace), then copy; a draw's error is returned after the copy.
 int hana_target_read(hana_ctx* c, hana_target* t, uint8_t* rgba, float* depth) {
     if (!c || !t) return HANA_E_INVALID;
+    int st = check_status(c);
+    if (st == HANA_E_DEVICE) return st;
     int r = join(c);
     if (r || (r = materialize_clear(c, t, c->stream))) return r;
     size_t n = (size_t)t->w * t->h;
     if (rgba) HIP_TRY(c, hipMemcpyAsync(rgba, t->color, 4 * n, hipMemcpyDeviceToHost, c->stream));
     if (depth) HIP_TRY(c, hipMemcpyAsync(depth, t->depth, 4 * n, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    return check_status(c);
+    return st;
 }
 
 int hana_target_present(hana_ctx* c, hana_target* t, uint8_t* bgra) {
     if (!c || !t || !bgra) return fail(c, HANA_E_INVALID, "hana_target_present: null argument");
+    int st = check_status(c);
+    if (st == HANA_E_DEVICE) return st;
     int r = join(c);
     if (r || (r = materialize_clear(c, t, c->stream))) return r;
     size_t n = (size_t)t->w * t->h;
@@ -877,7 +1022,7 @@ int hana_target_present(hana_ctx* c, hana_target* t, uint8_t* bgra) {
     HIP_TRY(c, launch_present(t->color, dst, t->w, t->h, c->stream));
     HIP_TRY(c, hipMemcpyAsync(bgra, dst, 4 * n, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    return check_status(c);
+    return st;
 }
 
 int hana_target_enable_winner(hana_ctx* c, hana_target* t, int enable) {
@@ -891,6 +1036,8 @@ int hana_target_enable_winner(hana_ctx* c, hana_target* t, int enable) {
         HIP_TRY(c, launch_fill32(t->winner, NO_WINNER, n, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));   // draws of either slot may use it next
     } else if (!enable && t->winner) {
+        int st = check_status(c);
+        if (st) return st;
         (void)sync_all(c);
         (void)hipFree(t->winner);
         t->winner = nullptr;
@@ -901,11 +1048,13 @@ int hana_target_enable_winner(hana_ctx* c, hana_target* t, int enable) {
 int hana_target_read_winner(hana_ctx* c, hana_target* t, uint32_t* winner) {
     if (!c || !t || !winner || !t->winner) return HANA_E_INVALID;
     size_t n = (size_t)t->w * t->h;
+    int st = check_status(c);
+    if (st == HANA_E_DEVICE) return st;
     int r = join(c);
     if (r) return r;
     HIP_TRY(c, hipMemcpyAsync(winner, t->winner, 4 * n, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    return HANA_OK;
+    return st;
 }
 
 int hana_draw(hana_ctx* c, const hana_draw_desc* d) {
@@ -978,21 +1127,15 @@ static void batch_drop_pending(hana_batch* b) {
     for (auto& s : b->specs) s.target->pending_clear = false;
 }
 
-// Parts of a batch.  Four measured best on 256 mixed scenes (2.10 ms per
-// batch, against 2.20 for two and 2.3-2.4 for six or eight): a process gets
-// four hardware queues, and more streams than that share them.
-#ifndef HANA_BATCH_PARTS
-#define HANA_BATCH_PARTS 4
-#endif
-constexpr size_t kBatchParts = HANA_BATCH_PARTS;
-
-// Splits the frames into up to kBatchParts parts of about equal work (faces
-// plus an eighth of the pixels drawn), at most one part per frame.
+// Splits the frames into up to b->tuning.parts parts of about equal work
+// (faces plus an eighth of the pixels drawn), at most one part per frame.
+// Four parts measured best on 256 mixed scenes (2.10 ms per batch, against
+// 2.20 for two and 2.3-2.4 for six or eight): a process gets four hardware
+// queues, and more streams than that share them.
 static void batch_split(hana_batch* b) {
     size_t nf = b->frame_first.size();
-    for (auto& part : b->parts) free_arena(part.arena);   // from a finalize that failed
     b->parts.clear();
-    size_t np = std::min(nf, kBatchParts);
+    size_t np = std::min(nf, std::min((size_t)b->tuning.parts, kMaxBatchParts));
     std::vector<double> w(nf, 0.0);
     double total = 0.0;
     for (size_t k = 0; k < nf; k++) {
@@ -1017,6 +1160,7 @@ static void batch_split(hana_batch* b) {
         }
         size_t s0 = b->frame_first[k0], s1 = k < nf ? b->frame_first[k] : b->specs.size();
         b->parts[p].specs.assign(b->specs.begin() + s0, b->specs.begin() + s1);
+        b->parts[p].arena = &b->arenas[p];
     }
 }
 
@@ -1034,7 +1178,7 @@ static int batch_enqueue(hana_ctx* c, hana_batch* b, bool timed) {
     int r;
     if (b->parts.size() == 1 || (timed && c->timing)) {
         for (auto& part : b->parts)
-            if ((r = enqueue_sequence(c, part.plan, part.arena, c->stream, timed))) return r;
+            if ((r = enqueue_sequence(c, part.plan, *part.arena, c->stream, timed))) return r;
         return HANA_OK;
     }
     HIP_TRY(c, hipEventRecord(b->fork, c->stream));
@@ -1043,7 +1187,7 @@ static int batch_enqueue(hana_ctx* c, hana_batch* b, bool timed) {
     }
     for (size_t p = 0; p < b->parts.size(); p++) {
         hipStream_t s = part_stream(c, b, p);
-        if ((r = enqueue_sequence(c, b->parts[p].plan, b->parts[p].arena, s, timed))) return r;
+        if ((r = enqueue_sequence(c, b->parts[p].plan, *b->parts[p].arena, s, timed))) return r;
     }
     for (size_t p = 1; p < b->parts.size(); p++) {
         hipStream_t s = part_stream(c, b, p);
@@ -1056,26 +1200,33 @@ static int batch_enqueue(hana_ctx* c, hana_batch* b, bool timed) {
 // Sizes a part's workspace exactly with trial runs.
 static int batch_size_part(hana_ctx* c, hana_batch* b, hana_batch_part& part) {
     int r;
-    if ((r = build_plan(c, part.specs, part.plan))) return r;
+    if ((r = build_plan(c, part.specs, part.plan, b->tuning))) return r;
     uint32_t min_bins = 0, min_polys = 0, min_tris = 0;
+    Arena& A = *part.arena;
     for (int attempt = 0; attempt < 3; attempt++) {
-        if ((r = bind_plan(c, part.plan, part.arena, min_bins, min_polys, min_tris, false, c->stream))) return r;
+        if ((r = bind_plan(c, part.plan, A, min_bins, min_polys, min_tris, false, c->stream))) return r;
         batch_clear_flags(part);
-        HIP_TRY(c, hipMemcpy(part.arena.tables.p, part.plan.descs.data(), sizeof(PassDesc) * part.plan.descs.size(),
+        HIP_TRY(c, hipMemcpy(A.tables.p, part.plan.descs.data(), sizeof(PassDesc) * part.plan.descs.size(),
                              hipMemcpyHostToDevice));
-        HIP_TRY(c, hipMemcpy(part.arena.tables.as<char>() + part.plan.uni_off, part.plan.unis.data(),
+        HIP_TRY(c, hipMemcpy(A.tables.as<char>() + part.plan.uni_off, part.plan.unis.data(),
                              sizeof(DevUniforms) * part.plan.unis.size(), hipMemcpyHostToDevice));
         // trial run: sizes the bins and the clip pool exactly
-        if ((r = enqueue_sequence(c, part.plan, part.arena, c->stream))) return r;
+        if ((r = enqueue_sequence(c, part.plan, A, c->stream))) return r;
         uint32_t st[STATUS_WORDS];
-        HIP_TRY(c, hipMemcpyAsync(st, part.arena.status.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(c, hipMemcpyAsync(st, A.status.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(c, hipStreamSynchronize(c->stream));
         batch_drop_pending(b);
         if (!st[STATUS_FLAGS]) return HANA_OK;
-        HIP_TRY(c, launch_fill32(part.arena.status.as<uint32_t>() + STATUS_SEQ_WORDS, 0u,
-                                 STATUS_WORDS - STATUS_SEQ_WORDS, c->stream));
-        if (st[STATUS_FLAGS] & FLAG_POLY_TOO_LARGE)
-            return fail(c, HANA_E_OVERFLOW, "hana_batch_finalize: a clipped polygon exceeded 10 vertices");
+        HIP_TRY(c, launch_fill32(A.status.as<uint32_t>() + STATUS_SEQ_WORDS, 0u, STATUS_WORDS - STATUS_SEQ_WORDS,
+                                 c->stream));
+        if (st[STATUS_FLAGS] & FLAG_QUEUE_STALL)
+            return fail(c, HANA_E_DEVICE, "hana_batch_finalize: the persistent raster's wait timed out");
+        if (!(st[STATUS_FLAGS] & FLAG_CAPACITY)) {
+            if (st[STATUS_FLAGS] & FLAG_PAIR_LIMIT)
+                return fail(c, HANA_E_NOMEM, "hana_batch_finalize: a part covers more than 4,294,967,280 "
+                                             "(triangle, tile) pairs; split the batch");
+            return HANA_OK;   // a polygon over POLY_MAX vertices is dropped at every run, like a per-call draw
+        }
         min_bins = st[STATUS_BINS_NEEDED] + st[STATUS_BINS_NEEDED] / 8 + 1024;
         min_polys = st[STATUS_POLYS] + st[STATUS_POLYS] / 8 + 1024;
         min_tris = st[STATUS_TRIS] + st[STATUS_TRIS] / 8 + 1024;
@@ -1083,16 +1234,17 @@ static int batch_size_part(hana_ctx* c, hana_batch* b, hana_batch_part& part) {
     return fail(c, HANA_E_OVERFLOW, "hana_batch_finalize: capacity not reached");
 }
 
-int hana_batch_finalize(hana_ctx* c, hana_batch* b, int use_graph) {
-    if (!c || !b) return HANA_E_INVALID;
-    if (b->specs.empty()) return fail(c, HANA_E_INVALID, "hana_batch_finalize: empty batch");
-    if (b->finalized) return fail(c, HANA_E_INVALID, "hana_batch_finalize: batch already finalized");
-    for (size_t i = 0; i < b->specs.size(); i++)
-        for (size_t j = i + 1; j < b->specs.size(); j++)
-            if (b->specs[i].target == b->specs[j].target)
-                return fail(c, HANA_E_INVALID, "hana_batch_finalize: frames of a batch need distinct targets and shadow maps");
+static void batch_drop_graph(hana_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// Splits, sizes and (with use_graph) captures the batch for b->tuning.
+static int batch_prepare(hana_ctx* c, hana_batch* b, bool use_graph) {
     int r;
-    if ((r = check_status(c)) || (r = join(c))) return r;
+    batch_drop_graph(b);
     batch_split(b);
     for (auto& part : b->parts)
         if ((r = batch_size_part(c, b, part))) return r;
@@ -1107,7 +1259,7 @@ int hana_batch_finalize(hana_ctx* c, hana_batch* b, int use_graph) {
         HIP_TRY(c, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         b->extra.push_back(s);
     }
-    b->use_graph = use_graph != 0;
+    b->use_graph = use_graph;
     if (b->use_graph) {
         HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         int er = batch_enqueue(c, b, false);
@@ -1121,12 +1273,31 @@ int hana_batch_finalize(hana_ctx* c, hana_batch* b, int use_graph) {
         b->graph = g;
         HIP_TRY(c, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     }
+    return HANA_OK;
+}
+
+static int batch_check_new(hana_ctx* c, hana_batch* b, const char* what) {
+    if (b->specs.empty()) return fail(c, HANA_E_INVALID, std::string(what) + ": empty batch");
+    if (b->finalized) return fail(c, HANA_E_INVALID, std::string(what) + ": batch already finalized");
+    for (size_t i = 0; i < b->specs.size(); i++)
+        for (size_t j = i + 1; j < b->specs.size(); j++)
+            if (b->specs[i].target == b->specs[j].target)
+                return fail(c, HANA_E_INVALID, std::string(what) + ": frames of a batch need distinct targets and shadow maps");
+    return HANA_OK;
+}
+
+int hana_batch_finalize(hana_ctx* c, hana_batch* b, int use_graph) {
+    if (!c || !b) return HANA_E_INVALID;
+    int r;
+    if ((r = batch_check_new(c, b, "hana_batch_finalize"))) return r;
+    if ((r = check_status(c)) || (r = join(c))) return r;
+    if ((r = batch_prepare(c, b, use_graph != 0))) return r;
     b->finalized = true;
     return HANA_OK;
 }
 
-int hana_batch_run(hana_ctx* c, hana_batch* b) {
-    if (!c || !b || !b->finalized) return fail(c, HANA_E_INVALID, "hana_batch_run: batch not finalized");
+// One run of a prepared batch on the context stream.
+static int batch_launch(hana_ctx* c, hana_batch* b) {
     batch_drop_pending(b);
     int r = join(c);
     if (r) return r;
@@ -1144,12 +1315,118 @@ int hana_batch_run(hana_ctx* c, hana_batch* b) {
     return HANA_OK;
 }
 
+// Mean time of `reps` runs after one untimed run, with events on the context
+// stream (every part joins it).
+static int batch_time(hana_ctx* c, hana_batch* b, int reps, double* ms) {
+    int r;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(c, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(c, HANA_E_DEVICE, "hipEventCreate");
+    }
+    float t = 0.f;
+    r = batch_launch(c, b);
+    if (!r) r = sync_all(c);
+    if (!r && hipEventRecord(e0, c->stream) != hipSuccess) r = fail(c, HANA_E_DEVICE, "hipEventRecord");
+    for (int i = 0; i < reps && !r; i++) r = batch_launch(c, b);
+    if (!r && hipEventRecord(e1, c->stream) != hipSuccess) r = fail(c, HANA_E_DEVICE, "hipEventRecord");
+    if (!r) r = sync_all(c);
+    if (!r && hipEventElapsedTime(&t, e0, e1) != hipSuccess) r = fail(c, HANA_E_DEVICE, "hipEventElapsedTime");
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = (double)t / std::max(reps, 1);
+    return r;
+}
+
+
+static void tuning_out(const Tuning& t, double ms, hana_tuning* out) {
+    std::memset(out, 0, sizeof *out);
+    out->tile_size = tile_edge(t.tile_shift);
+    out->raster_wgs_per_cu = t.raster_wgs_per_cu;
+    out->parts = t.parts;
+    out->ms = ms;
+}
+
+int hana_gpu_set_tuning(hana_ctx* c, const hana_tuning* t) {
+    if (!c) return HANA_E_INVALID;
+    Tuning tu;
+    int r = tuning_from(c, t, tu);
+    if (r) return r;
+    c->tuning = tu;
+    return HANA_OK;
+}
+
+int hana_batch_set_tuning(hana_ctx* c, hana_batch* b, const hana_tuning* t) {
+    if (!c || !b) return HANA_E_INVALID;
+    if (b->finalized) return fail(c, HANA_E_INVALID, "hana_batch_set_tuning: batch already finalized");
+    return tuning_from(c, t, b->tuning);
+}
+
+int hana_batch_get_tuning(hana_ctx* c, const hana_batch* b, hana_tuning* out) {
+    if (!c || !b || !out) return HANA_E_INVALID;
+    tuning_out(b->tuning, b->tuned_ms, out);
+    return HANA_OK;
+}
+
+// The candidates, one dimension at a time from the batch's current tuning:
+// the tile size, then the raster form (per-tile workgroups or a persistent
+// grid of 2, 4 or 8 workgroups per CU), then the number of parts; each keeps
+// the best value found so far for the others.
+int hana_batch_autotune(hana_ctx* c, hana_batch* b, int use_graph, int reps, hana_tuning* chosen) {
+    if (!c || !b || reps < 1) return HANA_E_INVALID;
+    int r;
+    if ((r = batch_check_new(c, b, "hana_batch_autotune"))) return r;
+    if ((r = check_status(c)) || (r = join(c))) return r;
+    Tuning best = b->tuning;
+    double best_ms = 0.0;
+    auto measure = [&](const Tuning& t, double* ms) {
+        b->tuning = t;
+        int e = batch_prepare(c, b, use_graph != 0);
+        return e ? e : batch_time(c, b, reps, ms);
+    };
+    if ((r = measure(best, &best_ms))) return r;
+    const int shifts[] = {4, 5}, wgs[] = {0, 2, 4, 8}, parts[] = {1, 2, 4};
+    for (int dim = 0; dim < 3; dim++) {
+        const int* vals = dim == 0 ? shifts : (dim == 1 ? wgs : parts);
+        int nv = dim == 0 ? 2 : (dim == 1 ? 4 : 3);
+        Tuning base = best;
+        for (int i = 0; i < nv; i++) {
+            Tuning t = base;
+            int& field = dim == 0 ? t.tile_shift : (dim == 1 ? t.raster_wgs_per_cu : t.parts);
+            const int cur = dim == 0 ? base.tile_shift : (dim == 1 ? base.raster_wgs_per_cu : base.parts);
+            if (vals[i] == cur) continue;
+            field = vals[i];
+            double ms = 0.0;
+            if ((r = measure(t, &ms))) return r;
+            if (ms < best_ms) {
+                best = t;
+                best_ms = ms;
+            }
+        }
+    }
+    b->tuning = best;
+    if ((r = batch_prepare(c, b, use_graph != 0))) return r;
+    b->tuned_ms = best_ms;
+    b->finalized = true;
+    if (chosen) tuning_out(best, best_ms, chosen);
+    return HANA_OK;
+}
+
+int hana_batch_run(hana_ctx* c, hana_batch* b) {
+    if (!c || !b || !b->finalized) return fail(c, HANA_E_INVALID, "hana_batch_run: batch not finalized");
+    // per-call draws before the batch are complete (replayed if need be)
+    // before the batch draws over their targets
+    int r = check_status(c);
+    if (r) return r;
+    return batch_launch(c, b);
+}
+
 int hana_batch_destroy(hana_ctx* c, hana_batch* b) {
     if (!c || !b) return HANA_E_INVALID;
     (void)sync_all(c);
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    for (auto& part : b->parts) free_arena(part.arena);
+    batch_drop_graph(b);
+    for (auto& a : b->arenas) free_arena(a);
     if (b->fork) (void)hipEventDestroy(b->fork);
     for (hipEvent_t e : b->joined) (void)hipEventDestroy(e);
     for (hipStream_t s : b->extra) (void)hipStreamDestroy(s);

@@ -154,7 +154,8 @@ int launch(hana_server* s, Lane& L) {
     for (Req& q : L.batch) {
         if (!q.target) q.target = get_target(s, q.r.width, q.r.height);
         if (q.r.enable_shadow && !q.shadow) q.shadow = get_target(s, q.sw, q.sh);
-        if (!q.host) q.host = get_pinned(s, (size_t)q.r.width * q.r.height * 4, &q.host_cap);
+        size_t img = (size_t)q.r.width * q.r.height * 4;
+        if (!q.host) q.host = get_pinned(s, (s->cfg.flags & HANA_SERVER_RETURN_DEPTH) ? 2 * img : img, &q.host_cap);
         if (!q.target || (q.r.enable_shadow && !q.shadow) || !q.host) return HANA_E_NOMEM;
         q.target->pending_clear = true;
         if (q.shadow) q.shadow->pending_clear = true;
@@ -167,10 +168,14 @@ int launch(hana_server* s, Lane& L) {
     }
     int rc = run_passes(L.ctx, std::move(specs));
     if (rc) return rc;
-    for (Req& q : L.batch)
-        if (hipMemcpyAsync(q.host, q.target->color, (size_t)q.r.width * q.r.height * 4, hipMemcpyDeviceToHost,
-                           L.ctx->stream) != hipSuccess)
+    for (Req& q : L.batch) {
+        size_t img = (size_t)q.r.width * q.r.height * 4;
+        if (hipMemcpyAsync(q.host, q.target->color, img, hipMemcpyDeviceToHost, L.ctx->stream) != hipSuccess)
             return HANA_E_DEVICE;
+        if ((s->cfg.flags & HANA_SERVER_RETURN_DEPTH) &&
+            hipMemcpyAsync(q.host + img, q.target->depth, img, hipMemcpyDeviceToHost, L.ctx->stream) != hipSuccess)
+            return HANA_E_DEVICE;
+    }
     if (hipEventRecord(L.done, L.ctx->stream) != hipSuccess) return HANA_E_DEVICE;
     L.busy = true;
     return HANA_OK;
@@ -188,6 +193,9 @@ void complete(hana_server* s, Lane& L, int status) {
         c.height = q.r.height;
         c.batch_frames = (int)L.batch.size();
         c.rgba = status == HANA_OK ? q.host : nullptr;
+        c.depth = status == HANA_OK && (s->cfg.flags & HANA_SERVER_RETURN_DEPTH)
+                      ? reinterpret_cast<const float*>(q.host + (size_t)q.r.width * q.r.height * 4)
+                      : nullptr;
         c.t_submit_ms = q.t_submit;
         c.t_launch_ms = q.t_launch;
         c.t_done_ms = t;
@@ -302,7 +310,7 @@ extern "C" {
 
 int hana_server_create(hana_ctx* ctx, const hana_server_config* cfg, hana_server** out) {
     if (!ctx || !cfg || !out || cfg->lanes < 1 || cfg->lanes > 8 || cfg->max_batch_frames < 1 ||
-        cfg->max_batch_pixels < 1 || cfg->hbm_budget_bytes < 1)
+        cfg->max_batch_pixels < 1 || cfg->hbm_budget_bytes < 1 || (cfg->flags & ~HANA_SERVER_RETURN_DEPTH))
         return fail(ctx, HANA_E_INVALID, "hana_server_create: bad configuration");
     hana_server* s = new hana_server();
     s->device = ctx->device;
@@ -324,8 +332,10 @@ int hana_server_create(hana_ctx* ctx, const hana_server_config* cfg, hana_server
             return fail(ctx, HANA_E_DEVICE, "hana_server_create: lane set-up failed");
         }
         // a lane's completion event is recorded on its context stream, so its
-        // draws stay on that one stream
+        // draws stay on that one stream; a batch that outgrew the workspace is
+        // launched again by the worker (with its image copies)
         L.ctx->pipelined = false;
+        L.ctx->replay_overflow = false;
         s->lanes.push_back(L);
     }
     s->worker = std::thread(worker_main, s);
@@ -345,6 +355,7 @@ int hana_server_submit(hana_server* s, const hana_request* r, uint64_t* id) {
     float aspect = (float)r->width / (float)r->height;
     build_uniforms(r->cam_pos, r->cam_target, aspect, r->light_pos, zero, zero, one, r->width, r->height,
                    r->enable_shadow, &q.u);
+    q.u.shadow_fp32 = r->shadow_fp32 ? 1 : 0;
     q.bytes = request_bytes(*r, q.sw, q.sh);
     {
         std::lock_guard<std::mutex> g(s->mu);

@@ -15,9 +15,12 @@ enum {
     SHADER_SHADOW = 6
 };
 
-constexpr int TILE = 16;                   // screen tile edge (pixels)
-constexpr int TILE_SHIFT = 4;
-constexpr int TILE_PIX = TILE * TILE;
+// Screen tiles are 2^shift pixels a side: 16x16 (shift 4) by default, 32x32
+// (shift 5) where a batch's tuning picks it.  Every kernel that maps pixels
+// to tiles is instantiated for both.
+constexpr int TILE_SHIFT_DEFAULT = 4;
+constexpr int TILE_SHIFT_MAX = 5;
+__host__ __device__ constexpr int tile_edge(int shift) { return 1 << shift; }
 constexpr int BLOCK = 256;                 // threads per workgroup (4 wave64)
 #ifndef HANA_RASTER_BLOCK
 #define HANA_RASTER_BLOCK 256
@@ -36,7 +39,7 @@ constexpr int BLOCK = 256;                 // threads per workgroup (4 wave64)
 #define HANA_ER_MIN_ENTRIES 320
 #endif
 constexpr int RBLOCK = HANA_RASTER_BLOCK;  // threads per raster workgroup (one tile)
-static_assert(RBLOCK % 64 == 0 && RBLOCK <= TILE_PIX, "raster workgroup: whole waves, at most one lane per pixel");
+static_assert(RBLOCK % 64 == 0 && RBLOCK <= 256, "raster workgroup: whole waves, at most one lane per pixel of a 16x16 tile");
 static_assert(HANA_SMALL_PIX >= 1 && HANA_SMALL_PIX <= 32, "small-triangle candidates fit a 32-bit mask");
 #ifndef HANA_VPT
 #define HANA_VPT 4
@@ -46,7 +49,12 @@ static_assert(HANA_SMALL_PIX >= 1 && HANA_SMALL_PIX <= 32, "small-triangle candi
 #endif
 constexpr int VPT = HANA_VPT;              // vertices per thread in k_vertex
 constexpr int FPT = HANA_FPT;              // faces per thread in k_setup
-constexpr int POLY_MAX = 10;               // vertices of a clipped polygon (3 + 7 planes)
+// Vertices the device keeps of a clipped polygon: a convex polygon gains at
+// most one vertex per plane (3 + 7), rounding in the inside tests can add a
+// few more; one lane of a 16-lane clip group holds each.
+constexpr int POLY_MAX = 16;
+constexpr int FAN_BITS = 4;                // fan index of a clipped triangle in its bin entry
+constexpr uint32_t POOL_CAP_MAX = (1u << (31 - FAN_BITS)) - 1u;   // polygons a workspace may hold
 constexpr int POLY_VARY = 8;               // varyings stored per clipped vertex
 constexpr uint32_t NO_WINNER = 0xFFFFFFFFu;
 constexpr uint32_t CLIPPED_BIT = 0x80000000u;
@@ -70,7 +78,7 @@ struct DevUniforms {
     float gloss;
     float bump_scale;
     int32_t enable_shadow;
-    int32_t pad_;
+    int32_t shadow_fp32;         // fp32 shadow-map extension (hana_uniforms)
 };
 
 // One pass = one graphics_draw_triangle call over a mesh into a target.
@@ -86,8 +94,10 @@ struct PassDesc {
     // textures: RGBA8 words in TGAColor byte order
     const uint32_t* diffuse; int32_t dw, dh;
     const uint32_t* normal;  int32_t nw, nh;
-    // shadow map sampled by the fragment program (RGBA8 target words)
+    // shadow map sampled by the fragment program (RGBA8 target words, and
+    // its fp32 depth for the fp32 extension)
     const uint32_t* shadow; int32_t sw, sh;
+    const float* shadow_depth;
     // target
     uint32_t* color; float* depth; uint32_t* winner;
     int32_t width, height, tiles_x, tiles_y;
@@ -132,6 +142,14 @@ struct Workspace {
     uint32_t clipq_cap;
     uint32_t tris_cap;
     uint32_t seg_region;         // records reserved for the setup segments
+    // queue of the persistent raster (k_raster_q): its items are the tiles of
+    // every pass in global tile order (shadow passes first, then main passes
+    // that sample no map drawn here, then those that do)
+    uint32_t* qstate;            // [0] next item, [1] group-0 tiles finished and published
+    uint32_t q_items;            // tiles of all passes
+    uint32_t q_g0;               // tiles of group-0 passes
+    uint32_t q_dep_from;         // first item of a pass that samples a map drawn by group 0
+    int32_t tile_shift;          // tiles are 2^tile_shift pixels a side
 };
 
 // the tile scan: SCAN_CHUNK tile counters per workgroup of SCAN_THREADS
@@ -139,16 +157,22 @@ constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_PER_THREAD = 16;
 constexpr int SCAN_CHUNK = SCAN_THREADS * SCAN_PER_THREAD;
 
-// status words.  The first STATUS_SEQ_WORDS are counters of one launch
-// sequence (zeroed by its first kernel): polygons allocated, faces queued for
-// clipping, clipped fan triangles listed, bin entries needed.  The others
-// stay set across sequences until the host reads them at a sync point: the
-// overflow flags and the largest needs seen, so that calls need not wait for
-// one another.
-enum { STATUS_POLYS = 0, STATUS_CLIPQ = 1, STATUS_TRIS = 2, STATUS_BINS_NEEDED = 3, STATUS_SEQ_WORDS = 4,
-       STATUS_FLAGS = 4, STATUS_NEED_BINS = 5, STATUS_NEED_POLYS = 6, STATUS_NEED_TRIS = 7, STATUS_WORDS = 8 };
+// status words.  The first STATUS_SEQ_WORDS belong to one launch sequence
+// (zeroed by its first kernel): polygons allocated, faces queued for
+// clipping, clipped fan triangles listed, bin entries needed, and the flags
+// this sequence raised.  The others stay set across sequences until the host
+// reads them at a sync point: the flags of every sequence and the largest
+// needs seen, so that calls need not wait for one another.
+enum { STATUS_POLYS = 0, STATUS_CLIPQ = 1, STATUS_TRIS = 2, STATUS_BINS_NEEDED = 3, STATUS_SEQ_FLAGS = 4,
+       STATUS_SEQ_WORDS = 5, STATUS_FLAGS = 5, STATUS_NEED_BINS = 6, STATUS_NEED_POLYS = 7, STATUS_NEED_TRIS = 8,
+       STATUS_WORDS = 9 };
 enum { FLAG_BIN_OVERFLOW = 1, FLAG_POLY_OVERFLOW = 2, FLAG_POLY_TOO_LARGE = 4, FLAG_CLIPQ_OVERFLOW = 8,
-       FLAG_TRIS_OVERFLOW = 16, FLAG_PAIR_LIMIT = 32 };
+       FLAG_TRIS_OVERFLOW = 16, FLAG_PAIR_LIMIT = 32, FLAG_QUEUE_STALL = 64 };
+// A workspace capacity ran out: the sequence is incomplete, its rasters write
+// nothing, and the host grows the workspace and renders it again.
+constexpr uint32_t FLAG_CAPACITY = FLAG_BIN_OVERFLOW | FLAG_POLY_OVERFLOW | FLAG_CLIPQ_OVERFLOW | FLAG_TRIS_OVERFLOW;
+// Flags of a sequence whose binning and rasters do nothing.
+constexpr uint32_t FLAG_NO_RASTER = FLAG_CAPACITY | FLAG_PAIR_LIMIT;
 // (triangle, tile) pairs one launch sequence may hold: bin positions are
 // 32-bit.  A sequence with more sets FLAG_PAIR_LIMIT and rasterises nothing.
 constexpr uint32_t MAX_PAIRS = 0xFFFFFFF0u;

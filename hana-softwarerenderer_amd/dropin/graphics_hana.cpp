@@ -20,7 +20,8 @@
 //
 // Errors: the CPU function has no error path.  A failure here (no device, a
 // shader subclass without a device program, a device error) prints its cause
-// to stderr and aborts; there is no CPU fallback.
+// to stderr and aborts; there is no CPU fallback.  A clipped polygon of more
+// than 16 vertices (possible only through rounding) is dropped with a warning.
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -190,15 +191,16 @@ void draw(DrawData* dd) {
     desc.normal = normal;
     desc.shadow_map = shadow;
     desc.target = tgt;
-    // a pass that outgrows the workspace is reported at the read; the library
-    // has grown it by then and the pass is run again from the same input
-    int rc = HANA_E_OVERFLOW;
-    for (int attempt = 0; attempt < 3 && rc == HANA_E_OVERFLOW; attempt++) {
-        check(ctx, hana_target_upload(ctx, tgt, rb->color_buffer, rb->depth_buffer), "target upload failed");
-        check(ctx, hana_draw(ctx, &desc), "draw failed");
-        rc = hana_target_read(ctx, tgt, d.rgba.data(), d.depth.data());
-    }
-    check(ctx, rc, "draw failed");
+    // a pass that outgrows the workspace is grown and rendered again inside
+    // the read; a clipped polygon past the device's 16 vertices is dropped
+    // with a warning (the reference's buffer holds 30)
+    check(ctx, hana_target_upload(ctx, tgt, rb->color_buffer, rb->depth_buffer), "target upload failed");
+    check(ctx, hana_draw(ctx, &desc), "draw failed");
+    int rc = hana_target_read(ctx, tgt, d.rgba.data(), d.depth.data());
+    if (rc == HANA_E_UNSUPPORTED)
+        std::fprintf(stderr, "graphics_draw_triangle: %s\n", hana_gpu_last_error(ctx));
+    else
+        check(ctx, rc, "draw failed");
     for (size_t i = 0; i < n; i++) std::memcpy(rb->color_buffer + 4 * i, d.rgba.data() + 4 * i, 3);
     std::memcpy(rb->depth_buffer, d.depth.data(), sizeof(float) * n);
     if (diffuse) hana_texture_destroy(ctx, diffuse);

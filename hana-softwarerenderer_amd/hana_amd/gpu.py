@@ -13,7 +13,7 @@ import numpy as np
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_DIR, "libhana_gpu.so")
 
-HANA_OK, HANA_E_INVALID, HANA_E_DEVICE, HANA_E_NOMEM, HANA_E_OVERFLOW = 0, -1, -2, -3, -4
+HANA_OK, HANA_E_INVALID, HANA_E_DEVICE, HANA_E_NOMEM, HANA_E_OVERFLOW, HANA_E_UNSUPPORTED = 0, -1, -2, -3, -4, -5
 GROUND, TOON, TEXTURE, TEXTURE_LIGHT, BLINN, NORMALMAP, SHADOW = range(7)
 FLT_MAX = float(np.finfo(np.float32).max)
 NO_WINNER = 0xFFFFFFFF
@@ -29,7 +29,7 @@ class Uniforms(C.Structure):
                 ("view_pos", C.c_float * 3), ("light_dir", C.c_float * 3), ("light_color", C.c_float * 4),
                 ("ambient", C.c_float * 4), ("mat_color", C.c_float * 4), ("mat_specular", C.c_float * 4),
                 ("gloss", C.c_float), ("bump_scale", C.c_float), ("enable_shadow", C.c_int32),
-                ("pad_", C.c_int32)]
+                ("shadow_fp32", C.c_int32)]
 
 
 class DrawDesc(C.Structure):
@@ -51,22 +51,35 @@ class PassStats(C.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
+SERVER_RETURN_DEPTH = 1
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("lanes", C.c_int32), ("max_batch_frames", C.c_int32), ("max_batch_pixels", C.c_int64),
-                ("hbm_budget_bytes", C.c_int64)]
+                ("hbm_budget_bytes", C.c_int64), ("flags", C.c_int32), ("pad_", C.c_int32)]
 
 
 class Request(C.Structure):
     _fields_ = [("mesh", _P), ("diffuse", _P), ("normal", _P), ("shader", C.c_int32), ("enable_shadow", C.c_int32),
                 ("width", C.c_int32), ("height", C.c_int32), ("shadow_width", C.c_int32),
                 ("shadow_height", C.c_int32), ("cam_pos", C.c_float * 3), ("cam_target", C.c_float * 3),
-                ("light_pos", C.c_float * 3), ("pad_", C.c_int32), ("user_tag", C.c_uint64)]
+                ("light_pos", C.c_float * 3), ("shadow_fp32", C.c_int32), ("user_tag", C.c_uint64)]
 
 
 class Completion(C.Structure):
     _fields_ = [("id", C.c_uint64), ("user_tag", C.c_uint64), ("status", C.c_int32), ("width", C.c_int32),
                 ("height", C.c_int32), ("batch_frames", C.c_int32), ("rgba", C.POINTER(C.c_uint8)),
-                ("t_submit_ms", C.c_double), ("t_launch_ms", C.c_double), ("t_done_ms", C.c_double)]
+                ("t_submit_ms", C.c_double), ("t_launch_ms", C.c_double), ("t_done_ms", C.c_double),
+                ("depth", C.POINTER(C.c_float))]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("tile_size", C.c_int32), ("raster_wgs_per_cu", C.c_int32), ("parts", C.c_int32),
+                ("pad_", C.c_int32), ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {"tile_size": int(self.tile_size), "raster_wgs_per_cu": int(self.raster_wgs_per_cu),
+                "parts": int(self.parts), "ms": float(self.ms)}
 
 
 class ServerStats(C.Structure):
@@ -95,6 +108,7 @@ DECLS = [
     ("hana_gpu_sync", C.c_int, [_P]),
     ("hana_gpu_set_stats", C.c_int, [_P, C.c_int]),
     ("hana_gpu_last_stats", C.c_int, [_P, C.POINTER(PassStats), C.c_int, C.POINTER(C.c_int)]),
+    ("hana_gpu_replay_count", C.c_int, [_P, C.POINTER(C.c_int64)]),
     ("hana_gpu_set_timing", C.c_int, [_P, C.c_int]),
     ("hana_gpu_read_timing", C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     ("hana_mesh_create", C.c_int, [_P, C.c_int32, _FP, _FP, _FP, C.c_int32, C.POINTER(C.c_int32), C.POINTER(_P)]),
@@ -111,7 +125,11 @@ DECLS = [
     ("hana_target_present", C.c_int, [_P, _P, C.POINTER(C.c_uint8)]),
     ("hana_draw", C.c_int, [_P, C.POINTER(DrawDesc)]),
     ("hana_render_frame", C.c_int, [_P, C.POINTER(FrameDesc)]),
+    ("hana_gpu_set_tuning", C.c_int, [_P, C.POINTER(Tuning)]),
     ("hana_batch_create", C.c_int, [_P, C.POINTER(_P)]),
+    ("hana_batch_set_tuning", C.c_int, [_P, _P, C.POINTER(Tuning)]),
+    ("hana_batch_autotune", C.c_int, [_P, _P, C.c_int, C.c_int, C.POINTER(Tuning)]),
+    ("hana_batch_get_tuning", C.c_int, [_P, _P, C.POINTER(Tuning)]),
     ("hana_batch_add_frame", C.c_int, [_P, _P, C.POINTER(FrameDesc)]),
     ("hana_batch_finalize", C.c_int, [_P, _P, C.c_int]),
     ("hana_batch_run", C.c_int, [_P, _P]),
@@ -255,6 +273,7 @@ def build_uniforms(scene, enable_shadow=None) -> Uniforms:
                                  _fp(arrs[5]), scene.width, scene.height, int(es), C.byref(u))
     if rc:
         raise HanaError(rc, "hana_build_uniforms")
+    u.shadow_fp32 = int(getattr(scene, "shadow_fp32", False))
     return u
 
 
@@ -349,7 +368,18 @@ class Context:
         self.check(self.lib.hana_gpu_last_stats(self.h, arr, max_passes, C.byref(n)))
         return [arr[i].as_dict() for i in range(n.value)]
 
+    def replay_count(self) -> int:
+        '''Waits that grew the workspace and rendered the pending draws again.'''
+        n = C.c_int64()
+        self.check(self.lib.hana_gpu_replay_count(self.h, C.byref(n)), "replay_count")
+        return int(n.value)
+
     STAGES = ("vertex", "setup", "clip", "scan", "bin", "unused", "raster_g0", "raster_g1")
+
+    def set_tuning(self, tile_size=16, raster_wgs_per_cu=0, parts=4):
+        '''Tuning of this context's per-call draws (hana_gpu_set_tuning).'''
+        t = Tuning(int(tile_size), int(raster_wgs_per_cu), int(parts), 0, 0.0)
+        self.check(self.lib.hana_gpu_set_tuning(self.h, C.byref(t)), "set_tuning")
 
     def set_timing(self, on: bool):
         self.check(self.lib.hana_gpu_set_timing(self.h, int(on)))
@@ -446,6 +476,22 @@ class Batch:
     def finalize(self, use_graph=True):
         self.ctx.check(self.ctx.lib.hana_batch_finalize(self.ctx.h, self.h, int(use_graph)), "batch_finalize")
 
+    def set_tuning(self, tile_size=16, raster_wgs_per_cu=0, parts=4):
+        t = Tuning(int(tile_size), int(raster_wgs_per_cu), int(parts), 0, 0.0)
+        self.ctx.check(self.ctx.lib.hana_batch_set_tuning(self.ctx.h, self.h, C.byref(t)), "batch_set_tuning")
+
+    def autotune(self, use_graph=True, reps=3) -> dict:
+        '''Finalizes with the fastest tuning found (hana_batch_autotune); returns it.'''
+        t = Tuning()
+        self.ctx.check(self.ctx.lib.hana_batch_autotune(self.ctx.h, self.h, int(use_graph), int(reps), C.byref(t)),
+                       "batch_autotune")
+        return t.as_dict()
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        self.ctx.check(self.ctx.lib.hana_batch_get_tuning(self.ctx.h, self.h, C.byref(t)), "batch_get_tuning")
+        return t.as_dict()
+
     def run(self):
         self.ctx.check(self.ctx.lib.hana_batch_run(self.ctx.h, self.h), "batch_run")
 
@@ -489,19 +535,20 @@ class Server:
     """Continuous-batching render server (hana_server_*)."""
 
     def __init__(self, ctx: Context, lanes=2, max_batch_frames=64, max_batch_pixels=64 << 20,
-                 hbm_budget_bytes=200 << 30):
+                 hbm_budget_bytes=200 << 30, return_depth=False):
         self.ctx = ctx
         self.lib = ctx.lib
-        cfg = ServerConfig(lanes, max_batch_frames, max_batch_pixels, hbm_budget_bytes)
+        cfg = ServerConfig(lanes, max_batch_frames, max_batch_pixels, hbm_budget_bytes,
+                           SERVER_RETURN_DEPTH if return_depth else 0, 0)
         h = _P()
         ctx.check(self.lib.hana_server_create(ctx.h, C.byref(cfg), C.byref(h)), "server_create")
         self.h = h
 
     def submit(self, mesh, shader, shadow, width, height, cam_pos, light_pos, diffuse=None, normal=None,
-               cam_target=(0.0, 0.0, 0.0), shadow_wh=(0, 0), tag=0) -> int:
+               cam_target=(0.0, 0.0, 0.0), shadow_wh=(0, 0), tag=0, shadow_fp32=False) -> int:
         r = Request(mesh, diffuse, normal, int(shader), int(shadow), int(width), int(height), int(shadow_wh[0]),
                     int(shadow_wh[1]), (C.c_float * 3)(*cam_pos), (C.c_float * 3)(*cam_target),
-                    (C.c_float * 3)(*light_pos), 0, int(tag))
+                    (C.c_float * 3)(*light_pos), int(bool(shadow_fp32)), int(tag))
         rid = C.c_uint64()
         rc = self.lib.hana_server_submit(self.h, C.byref(r), C.byref(rid))
         if rc:
@@ -520,6 +567,13 @@ class Server:
     def image(c: Completion) -> np.ndarray:
         n = c.width * c.height * 4
         return np.ctypeslib.as_array(c.rgba, shape=(n,)).reshape(c.height, c.width, 4).copy()
+
+    @staticmethod
+    def depth(c: Completion):
+        '''The completion's depth buffer (servers made with return_depth), else None.'''
+        if not c.depth:
+            return None
+        return np.ctypeslib.as_array(c.depth, shape=(c.width * c.height,)).reshape(c.height, c.width).copy()
 
     def release(self, rid: int):
         self.lib.hana_server_release(self.h, C.c_uint64(rid))

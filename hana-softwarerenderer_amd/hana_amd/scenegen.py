@@ -445,6 +445,7 @@ class Scene:
     obj_scale: tuple = (1.0, 1.0, 1.0)
     name: str = ""
     meta: dict = field(default_factory=dict)
+    shadow_fp32: bool = False           # the fp32 shadow-map extension (hana_uniforms::shadow_fp32)
 
     @property
     def aspect(self) -> float:

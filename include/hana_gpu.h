@@ -30,9 +30,13 @@
  * raster waits for every earlier draw that draws or samples one of its
  * targets or shadow maps, so each target sees the draws in call order; draws
  * on disjoint targets may run concurrently.  hana_target_read(_winner),
- * hana_target_present and hana_gpu_sync wait for all of it.  If a draw since
- * the previous such wait outgrew the workspace, the wait returns
- * HANA_E_OVERFLOW after growing it: render those frames again.
+ * hana_target_present and hana_gpu_sync wait for all of it.  A draw that
+ * outgrows the workspace (bins, clip pool) leaves its targets untouched; the
+ * next such wait grows the workspace and renders the draws since the previous
+ * wait again, in call order, before it returns (the bin overflow of
+ * SR/graphics.cpp has no counterpart: the reference draws without a bound).
+ * Uploads, batch runs and the destruction of objects a pending draw uses wait
+ * the same way first.
  */
 #ifndef HANA_GPU_H
 #define HANA_GPU_H
@@ -48,7 +52,9 @@ enum {
     HANA_E_INVALID = -1,     /* bad argument */
     HANA_E_DEVICE = -2,      /* HIP runtime error */
     HANA_E_NOMEM = -3,       /* device allocation failed or over budget */
-    HANA_E_OVERFLOW = -4     /* internal capacity exceeded after retry */
+    HANA_E_OVERFLOW = -4,    /* internal capacity exceeded after three replays */
+    HANA_E_UNSUPPORTED = -5  /* a clipped polygon had more than 16 vertices: it was
+                                dropped, the rest drawn (the reference keeps 30) */
 };
 
 /* IShader subclasses (SR/IShader.h:132-165). */
@@ -85,7 +91,15 @@ typedef struct hana_uniforms {
     float gloss;
     float bump_scale;
     int32_t enable_shadow;
-    int32_t pad_;
+    /* 0: the reference's shadow map (parity mode): the shadow pass writes
+     * White * clamp01(light clip z) as RGBA8 and is_in_shadow compares
+     * against R/255 (SR/IShader.cpp:176-180, SR/IShader.h:124).
+     * 1: fp32 extension (no reference counterpart): the shadow pass writes
+     * depth only, and is_in_shadow compares against 2*d - 1, the light clip z
+     * of the nearest caster from the map's fp32 depth d (an uncovered texel
+     * holds the clear depth and lights).  Set it alike for a frame's shadow
+     * and main passes. */
+    int32_t shadow_fp32;
 } hana_uniforms;
 
 /* One graphics_draw_triangle call (DrawData, SR/graphics.h:9-13). */
@@ -129,6 +143,10 @@ int hana_gpu_sync(hana_ctx* ctx);
 /* Collect hana_pass_stats for subsequent draws (adds a few atomics). */
 int hana_gpu_set_stats(hana_ctx* ctx, int enable);
 int hana_gpu_last_stats(hana_ctx* ctx, hana_pass_stats* out, int max_passes, int* n_passes);
+/* How many times a wait found a draw that outgrew the workspace and rendered
+ * the draws since the previous wait again (no counterpart in the reference,
+ * whose face loop has no capacity bound). */
+int hana_gpu_replay_count(hana_ctx* ctx, int64_t* out);
 
 /* Per-stage kernel timing with HIP events recorded around each launch
  * (benchmarking; while it is on, draws use one stream and do not overlap).  Stages: 0 vertex, 1 set-up (cull,
@@ -176,15 +194,44 @@ int hana_target_present(hana_ctx* ctx, hana_target* t, uint8_t* bgra);
 int hana_draw(hana_ctx* ctx, const hana_draw_desc* d);
 int hana_render_frame(hana_ctx* ctx, const hana_frame_desc* f);
 
+/* ---- tuning: how launch sequences are laid out ----
+ * No reference counterpart (the reference scans each triangle's bounding box
+ * on one core).  tile_size: screen tiles of 16x16 or 32x32 pixels (binning and
+ * raster).  raster_wgs_per_cu: 0 rasterises each pass group with one
+ * workgroup per tile; n > 0 runs one persistent raster of n workgroups per CU
+ * that takes the tiles of every pass from a queue (shadow passes first, main
+ * passes waiting for them without a second launch).  parts: the concurrent
+ * parts a batch is split into (1-4; ignored for per-call draws).  Results are
+ * identical under every tuning.  Default: 16, 0, 4. */
+typedef struct hana_tuning {
+    int32_t tile_size;
+    int32_t raster_wgs_per_cu;
+    int32_t parts;
+    int32_t pad_;
+    double ms;                      /* hana_batch_autotune: measured run time of the choice */
+} hana_tuning;
+/* Tuning of the context's per-call draws (hana_draw, hana_render_frame). */
+int hana_gpu_set_tuning(hana_ctx* ctx, const hana_tuning* t);
+
 /* ---- batches: many frames in one captured graph ----
  * The frames are split into up to four parts that run concurrently (branches
  * of the graph on the context's streams and the batch's own); they need
  * distinct targets and shadow maps.  hana_batch_run returns once the work is
- * enqueued. */
+ * enqueued; it first completes the per-call draws enqueued before it. */
 int hana_batch_create(hana_ctx* ctx, hana_batch** out);
 int hana_batch_add_frame(hana_ctx* ctx, hana_batch* b, const hana_frame_desc* f);
+/* Tuning used by hana_batch_finalize (before it). */
+int hana_batch_set_tuning(hana_ctx* ctx, hana_batch* b, const hana_tuning* t);
 /* Build the workspace and (when use_graph) capture the launch sequence. */
 int hana_batch_finalize(hana_ctx* ctx, hana_batch* b, int use_graph);
+/* Finalizes the batch with the tuning that ran it fastest: from the batch's
+ * tuning, tile size, then raster form (per tile, or 2, 4 or 8 persistent
+ * workgroups per CU), then parts (1, 2, 4), one dimension at a time, each
+ * candidate sized, captured and timed over `reps` runs.  The frames are
+ * rendered by every run.  `chosen` (nullable) receives the choice and its
+ * time. */
+int hana_batch_autotune(hana_ctx* ctx, hana_batch* b, int use_graph, int reps, hana_tuning* chosen);
+int hana_batch_get_tuning(hana_ctx* ctx, const hana_batch* b, hana_tuning* out);
 int hana_batch_run(hana_ctx* ctx, hana_batch* b);
 int hana_batch_destroy(hana_ctx* ctx, hana_batch* b);
 
@@ -198,11 +245,15 @@ int hana_batch_destroy(hana_ctx* ctx, hana_batch* b);
  * cleared targets; its colour image is copied back before completion. */
 typedef struct hana_server hana_server;
 
+enum { HANA_SERVER_RETURN_DEPTH = 1 };   /* completions carry the fp32 depth buffer too */
+
 typedef struct hana_server_config {
     int32_t lanes;                /* concurrent batches (streams), >= 1 */
     int32_t max_batch_frames;     /* frames per batch */
     int64_t max_batch_pixels;     /* frame + shadow pixels per batch */
     int64_t hbm_budget_bytes;     /* in-flight device footprint limit */
+    int32_t flags;                /* HANA_SERVER_* */
+    int32_t pad_;
 } hana_server_config;
 
 typedef struct hana_request {
@@ -216,7 +267,7 @@ typedef struct hana_request {
     float cam_pos[3];
     float cam_target[3];
     float light_pos[3];
-    int32_t pad_;
+    int32_t shadow_fp32;          /* hana_uniforms::shadow_fp32 */
     uint64_t user_tag;
 } hana_request;
 
@@ -228,6 +279,7 @@ typedef struct hana_completion {
     int32_t batch_frames;         /* size of the batch it ran in */
     const uint8_t* rgba;          /* W*H*4 image, valid until hana_server_release */
     double t_submit_ms, t_launch_ms, t_done_ms;   /* server clock */
+    const float* depth;           /* W*H depth (HANA_SERVER_RETURN_DEPTH), else NULL; same lifetime */
 } hana_completion;
 
 typedef struct hana_server_stats {

@@ -424,7 +424,11 @@ static int is_in_shadow(const ctx_t* c, const float* dp, float n_dot_l) {
         float current_depth = dp[2] - depth_bias;
         if (pos[0] < 0 || pos[1] < 0 || pos[0] >= widht || pos[1] >= height) return 1;
         int px = (int)pos[0], py = (int)pos[1];
-        float closest_depth = sm->color[((size_t)py * sm->width + px) * 4 + 0] / 255.f;
+        float closest_depth;
+        if (c->u->shadow_fp32)   /* extension: the nearest caster's light clip z from the fp32 depth */
+            closest_depth = 2.f * sm->depth[(size_t)py * sm->width + px] - 1.f;
+        else
+            closest_depth = sm->color[((size_t)py * sm->width + px) * 4 + 0] / 255.f;
         return current_depth < closest_depth;
     }
     return 1;
@@ -681,9 +685,13 @@ static void rasterize_triangle(draw_t* d, const ho_v2f* v, int32_t prim) {
                 float sum = s0[k] * w0 + s1[k] * w1 + s2[k] * w2;
                 dst[k] = sum * normalizer;
             }
-            rgba col = fragment_impl(d->c, d->kind, &iv);
             d->st->fragments++;
             rb->depth[idx] = z;                            /* RenderBuffer::set_depth */
+            if (d->kind == HO_SHADER_SHADOW && d->c->u->shadow_fp32) {   /* extension: depth-only shadow pass */
+                if (d->winner) d->winner[idx] = prim;
+                continue;
+            }
+            rgba col = fragment_impl(d->c, d->kind, &iv);
             unsigned char* cp = rb->color + idx * 4;       /* RenderBuffer::set_color (SR/renderbuffer.cpp:38-44) */
             cp[0] = (unsigned char)(col.r * 255);
             cp[1] = (unsigned char)(col.g * 255);

@@ -76,7 +76,7 @@ typedef struct {
     float gloss;
     float bump_scale;
     int32_t enable_shadow;
-    int32_t pad_;
+    int32_t shadow_fp32;   /* fp32 shadow-map extension (no reference counterpart): see is_in_shadow */
 } ho_uniforms;
 
 /* TGAImage storage (SR/tgaimage.h:66-98): bytespp in {1,3,4}, bytes in

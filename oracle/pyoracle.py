@@ -31,7 +31,7 @@ class Uniforms(C.Structure):
                 ("view_pos", C.c_float * 3), ("light_dir", C.c_float * 3), ("light_color", C.c_float * 4),
                 ("ambient", C.c_float * 4), ("mat_color", C.c_float * 4), ("mat_specular", C.c_float * 4),
                 ("gloss", C.c_float), ("bump_scale", C.c_float), ("enable_shadow", C.c_int32),
-                ("pad_", C.c_int32)]
+                ("shadow_fp32", C.c_int32)]
 
 
 class A2V(C.Structure):
@@ -322,6 +322,7 @@ class CPURenderer:
         es = scene.shadow if enable_shadow is None else enable_shadow
         self._build(vals[0][1], vals[1][1], np.float32(scene.width) / np.float32(scene.height), light[1],
                     op[1], orot[1], osc[1], scene.width, scene.height, int(es), C.byref(u))
+        u.shadow_fp32 = int(getattr(scene, "shadow_fp32", False))
         return u
 
     def vertex(self, kind, u: Uniforms, a2v: np.ndarray) -> np.ndarray:

@@ -1,6 +1,11 @@
 """Batched rendering: many scenes of uneven size in one launch sequence,
 captured in a hipGraph and replayed.  Each scene must equal the oracle (same
-bar as the per-frame path) and the per-frame GPU path bit for bit."""
+bar as the per-frame path) and the per-frame GPU path bit for bit.
+
+Finalisation renders every frame once (a trial run that sizes the
+workspace), so before each checked run every target and shadow map is filled
+with a sentinel (colour bytes 0x5A, NaN depth): what is compared was written
+by that run."""
 import numpy as np
 import pytest
 
@@ -37,6 +42,41 @@ def read_all(ctx, devs):
     return [d.read() for d in devs]
 
 
+def poison(ctx, devs):
+    """Fills every target and shadow map of the frames with the sentinel."""
+    for d in devs:
+        maps = [(d.target, d.scene.width, d.scene.height)]
+        if d.shadow_map is not None:
+            maps.append((d.shadow_map,) + tuple(d.scene.shadow_wh))
+        for t, w, h in maps:
+            ctx.upload(t, np.full((h, w, 4), 0x5A, np.uint8), np.full((h, w), np.nan, np.float32))
+
+
+def test_config3_full_batch(ctx):
+    """The whole batched workload: 256 scenes of 1-8 displaced spheres/boxes
+    (1k-200k triangles, 128^2-1024^2 pixels, random aspect, Blinn/NormalMap x
+    shadow on/off) in one captured graph, against the oracle frame by frame:
+    depth bit-exact, colour within the bar, on two runs."""
+    from hana_amd.gpu import Batch, SceneOnDevice
+    scenes = sg.config3_scenes()
+    assert len(scenes) == 256
+    o = CPURenderer("oracle")
+    refs = [o.render_frame(s)[0] for s in scenes]
+    devs = [SceneOnDevice(ctx, s) for s in scenes]
+    batch = Batch(ctx)
+    for d in devs:
+        batch.add(d.desc())
+    batch.finalize(use_graph=True)
+    for run in range(2):
+        poison(ctx, devs)
+        batch.run()
+        for s, d, ref in zip(scenes, devs, refs):
+            col, dep = d.read()
+            assert_depth_exact(dep, ref.depth, f"{s.name} run {run}")
+            assert_color_ok(col, ref.color, f"{s.name} run {run}")
+    batch.close()
+
+
 def test_batch_matches_oracle_and_per_frame(ctx, scenes, oracle_frames):
     from hana_amd.gpu import Batch, SceneOnDevice
     devs = [SceneOnDevice(ctx, s) for s in scenes]
@@ -44,6 +84,7 @@ def test_batch_matches_oracle_and_per_frame(ctx, scenes, oracle_frames):
     for d in devs:
         batch.add(d.desc())
     batch.finalize(use_graph=True)
+    poison(ctx, devs)
     batch.run()
     out_b = read_all(ctx, devs)
     for s, (col, dep), (tgt, _, _) in zip(scenes, out_b, oracle_frames):
@@ -68,6 +109,7 @@ def test_graph_replays(ctx, scenes, oracle_frames):
     batch.finalize(use_graph=True)
     first = None
     for rep in range(4):
+        poison(ctx, devs)
         batch.run()
         # eager work between replays: a per-frame render on another target
         extra = SceneOnDevice(ctx, scenes[6 + rep % 4])
@@ -92,6 +134,7 @@ def test_batch_without_graph(ctx, scenes, oracle_frames):
     for d in devs:
         batch.add(d.desc())
     batch.finalize(use_graph=False)
+    poison(ctx, devs)
     batch.run()
     for s, (col, dep), (tgt, _, _) in zip(scenes[:5], read_all(ctx, devs), oracle_frames[:5]):
         assert_depth_exact(dep, tgt.depth, s.name)
@@ -111,6 +154,7 @@ def test_small_batches(ctx, scenes, oracle_frames, pick, use_graph):
         batch.add(d.desc())
     batch.finalize(use_graph=use_graph)
     for _ in range(2):
+        poison(ctx, devs)
         batch.run()
         for i, (col, dep) in zip(pick, read_all(ctx, devs)):
             tgt = oracle_frames[i][0]
@@ -149,11 +193,51 @@ def test_empty_mesh_frames(ctx, empty_first):
             batch.add(d.desc())
         batch.finalize(use_graph=graph)
         for _ in range(2):
-            for d in devs:
-                d.clear()
+            poison(ctx, devs)
             batch.run()
             for s, d, ref in zip(scenes, devs, refs):
                 col, dep = d.read()
                 assert_depth_exact(dep, ref.depth, s.name)
                 assert_color_ok(col, ref.color, s.name)
         batch.close()
+
+
+@pytest.mark.parametrize("tile,wgs,parts", [(32, 0, 4), (16, 4, 2), (32, 8, 1), (16, 2, 4)])
+def test_batch_tunings(ctx, scenes, oracle_frames, tile, wgs, parts):
+    """Every batch tuning (tile size, persistent raster, parts) renders the
+    oracle's frames, on two graph runs."""
+    from hana_amd.gpu import Batch, SceneOnDevice
+    devs = [SceneOnDevice(ctx, s) for s in scenes]
+    batch = Batch(ctx)
+    for d in devs:
+        batch.add(d.desc())
+    batch.set_tuning(tile, wgs, parts)
+    batch.finalize(use_graph=True)
+    assert batch.tuning()["tile_size"] == tile
+    for run in range(2):
+        poison(ctx, devs)
+        batch.run()
+        for s, (col, dep), (tgt, _, _) in zip(scenes, read_all(ctx, devs), oracle_frames):
+            assert_depth_exact(dep, tgt.depth, f"{s.name} run {run}")
+            assert_color_ok(col, tgt.color, f"{s.name} run {run}")
+    batch.close()
+
+
+def test_autotune_picks_a_tuning_and_renders_exactly(ctx, scenes, oracle_frames):
+    """hana_batch_autotune times the candidates, finalizes with the fastest,
+    and the tuned batch still renders the oracle's frames."""
+    from hana_amd.gpu import Batch, SceneOnDevice
+    devs = [SceneOnDevice(ctx, s) for s in scenes]
+    batch = Batch(ctx)
+    for d in devs:
+        batch.add(d.desc())
+    chosen = batch.autotune(use_graph=True, reps=2)
+    assert chosen["tile_size"] in (16, 32) and chosen["raster_wgs_per_cu"] in (0, 2, 4, 8)
+    assert chosen["parts"] in (1, 2, 4) and chosen["ms"] > 0
+    assert batch.tuning() == chosen
+    poison(ctx, devs)
+    batch.run()
+    for s, (col, dep), (tgt, _, _) in zip(scenes, read_all(ctx, devs), oracle_frames):
+        assert_depth_exact(dep, tgt.depth, s.name)
+        assert_color_ok(col, tgt.color, s.name)
+    batch.close()

@@ -359,65 +359,101 @@ def straddling_scene(n=40_000):
 
 
 def test_clip_pool_overflow_regrows(oracle):
-    """The first render overflows the clip pool and reports HANA_E_OVERFLOW at
-    the read; the library has grown its buffers and the re-render is exact.
-    A context of its own, so that no earlier test has grown the pool."""
+    """The first render overflows the clip pool: the read grows the workspace
+    and renders the draw again by itself, and returns the exact image; a
+    second render needs no replay.  A context of its own, so that no earlier
+    test has grown the pool."""
     from hana_amd.gpu import Context
     with Context(0) as ctx:
         check_regrowth(ctx, oracle)
 
 
 def check_regrowth(ctx, oracle):
-    from hana_amd.gpu import HANA_E_OVERFLOW, HanaError, build_uniforms
+    from hana_amd.gpu import build_uniforms
     scene = straddling_scene()
     u = build_uniforms(scene)
     mesh, dif, nrm = ctx.mesh(scene.mesh), ctx.texture(scene.diffuse), ctx.texture(scene.normal)
     tgt = ctx.target(scene.width, scene.height, winner=True)
-    ctx.clear(tgt)
-    ctx.draw(mesh, scene.shader, u, tgt, dif, nrm)
-    with pytest.raises(HanaError) as err:
-        ctx.read(tgt, scene.width, scene.height)
-    assert err.value.code == HANA_E_OVERFLOW
-    ctx.clear(tgt)
-    ctx.draw(mesh, scene.shader, u, tgt, dif, nrm)
-    col, dep = ctx.read(tgt, scene.width, scene.height)
-    win = ctx.read_winner(tgt, scene.width, scene.height)
     ref, owin, _, st, _ = oracle_passes(oracle, scene)
     assert st["clipped_faces"] == 40_000
-    assert np.array_equal(win.astype(np.int64), as_u32_ids(owin))
-    assert_depth_exact(dep, ref.depth, "straddling")
-    assert_color_ok(col, ref.color, "straddling")
+    r0 = ctx.replay_count()
+    for attempt in range(2):
+        ctx.clear(tgt)
+        ctx.draw(mesh, scene.shader, u, tgt, dif, nrm)
+        col, dep = ctx.read(tgt, scene.width, scene.height)
+        win = ctx.read_winner(tgt, scene.width, scene.height)
+        assert ctx.replay_count() == r0 + 1, attempt   # the first draw overflowed, the second fits
+        assert np.array_equal(win.astype(np.int64), as_u32_ids(owin)), attempt
+        assert_depth_exact(dep, ref.depth, "straddling")
+        assert_color_ok(col, ref.color, "straddling")
+
+
+def test_overflow_replays_calls_in_order(oracle):
+    """Calls since the last wait are rendered again in call order when one of
+    them outgrew the workspace: an overflowing draw into a target, a draw on
+    top of it, a frame into a second target whose shadow map a later draw
+    samples after its post-frame clear, then the reads.  Every image equals
+    the oracle's for the same call sequence."""
+    from hana_amd.gpu import SHADOW, Context, build_uniforms
+    big = straddling_scene()
+    top = sg.small_scene(sg.TEXTURE, seed=21, width=big.width, height=big.height)
+    fr = sg.small_scene(sg.BLINN, shadow=True, seed=22, width=80, height=60)
+    with Context(0) as ctx:
+        ub, ut, uf = build_uniforms(big), build_uniforms(top), build_uniforms(fr)
+        t1 = ctx.target(big.width, big.height)
+        t2 = ctx.target(fr.width, fr.height)
+        sm = ctx.target(fr.width, fr.height)
+        r0 = ctx.replay_count()
+        ctx.draw(ctx.mesh(big.mesh), big.shader, ub, t1, ctx.texture(big.diffuse), ctx.texture(big.normal))
+        ctx.draw(ctx.mesh(top.mesh), top.shader, ut, t1, ctx.texture(top.diffuse), ctx.texture(top.normal))
+        fm, fd, fn = ctx.mesh(fr.mesh), ctx.texture(fr.diffuse), ctx.texture(fr.normal)
+        ctx.render_frame(fm, fr.shader, True, uf, t2, sm, fd, fn)
+        ctx.draw(fm, sg.BLINN, uf, t2, fd, fn, sm)   # samples the cleared shadow map
+        c1, d1 = ctx.read(t1, big.width, big.height)
+        c2, d2 = ctx.read(t2, fr.width, fr.height)
+        assert ctx.replay_count() == r0 + 1
+    o1 = HostTarget(big.width, big.height)
+    oracle.draw(big.mesh, big.shader, oracle.build_uniforms(big), o1, big.diffuse, big.normal)
+    oracle.draw(top.mesh, top.shader, oracle.build_uniforms(top), o1, top.diffuse, top.normal)
+    o2, osm = HostTarget(fr.width, fr.height), HostTarget(fr.width, fr.height)
+    ou = oracle.build_uniforms(fr)
+    oracle.draw(fr.mesh, SHADOW, ou, osm, fr.diffuse, fr.normal)
+    oracle.draw(fr.mesh, fr.shader, ou, o2, fr.diffuse, fr.normal, osm)
+    osm.clear((0.0, 0.0, 0.0, 1.0), float(np.finfo(np.float32).max))   # DrawModel::draw's post-frame clear
+    oracle.draw(fr.mesh, sg.BLINN, ou, o2, fr.diffuse, fr.normal, osm)
+    assert_depth_exact(d1, o1.depth, "overflowed target")
+    assert_color_ok(c1, o1.color, "overflowed target")
+    assert_depth_exact(d2, o2.depth, "second target")
+    assert_color_ok(c2, o2.color, "second target")
 
 
 def test_bin_overflow_regrows(oracle):
     """800 k slivers of six (triangle, tile) pairs each overflow the bins the
-    first draw allocates (four pairs per face): the read reports
-    HANA_E_OVERFLOW, the bins have grown, and the re-render is exact.  A
-    context of its own, so that no earlier test has grown the bins."""
-    from hana_amd.gpu import HANA_E_OVERFLOW, Context, HanaError, build_uniforms
+    first draw allocates (four pairs per face): the read grows the bins and
+    renders the draw again by itself, exactly; later draws on every draw slot
+    need no replay.  A context of its own, so that no earlier test has grown
+    the bins."""
+    from hana_amd.gpu import Context, build_uniforms
     scene = sliver_scene()
     W, H = scene.width, scene.height
+    ref = HostTarget(W, H)
+    st, owin = oracle.draw(scene.mesh, scene.shader, with_camera_vp(oracle.build_uniforms(scene), IDENTITY_VP),
+                           ref, scene.diffuse, scene.normal, winner=True)
+    assert st["fragments"] > 1_000_000
     with Context(0) as ctx:
         u = with_camera_vp(build_uniforms(scene), IDENTITY_VP)
         mesh, dif, nrm = ctx.mesh(scene.mesh), ctx.texture(scene.diffuse), ctx.texture(scene.normal)
         tgt = ctx.target(W, H, winner=True)
-        ctx.clear(tgt)
-        ctx.draw(mesh, scene.shader, u, tgt, dif, nrm)
-        with pytest.raises(HanaError) as err:
-            ctx.read(tgt, W, H)
-        assert err.value.code == HANA_E_OVERFLOW
-        for _ in range(2):                                   # both draw slots have grown
+        r0 = ctx.replay_count()
+        for attempt in range(3):                             # every draw slot has grown
             ctx.clear(tgt)
             ctx.draw(mesh, scene.shader, u, tgt, dif, nrm)
             col, dep = ctx.read(tgt, W, H)
             win = ctx.read_winner(tgt, W, H)
-        ref = HostTarget(W, H)
-        st, owin = oracle.draw(scene.mesh, scene.shader, with_camera_vp(oracle.build_uniforms(scene), IDENTITY_VP),
-                               ref, scene.diffuse, scene.normal, winner=True)
-        assert st["fragments"] > 1_000_000
-        assert np.array_equal(win.astype(np.int64), as_u32_ids(owin))
-        assert_depth_exact(dep, ref.depth, "slivers")
-        assert_color_ok(col, ref.color, "slivers")
+            assert ctx.replay_count() == r0 + 1, attempt
+            assert np.array_equal(win.astype(np.int64), as_u32_ids(owin)), attempt
+            assert_depth_exact(dep, ref.depth, "slivers")
+            assert_color_ok(col, ref.color, "slivers")
 
 
 def test_pair_limit_refused(oracle):
@@ -462,3 +498,78 @@ def test_clip_pool_overflow_in_a_batch(ctx, oracle):
     assert_depth_exact(dep, tgt.depth, "straddling batch")
     assert_color_ok(col, tgt.color, "straddling batch")
     batch.close()
+
+
+# Tunings of the per-call path: 32x32 tiles, the persistent raster (both pass
+# groups in one launch, main passes waiting for the shadow tiles), and both.
+TUNINGS = [(32, 0), (16, 4), (32, 8), (16, 1)]
+
+
+def frame_scenes():
+    return [sg.small_scene(sg.NORMALMAP, shadow=True, seed=1),
+            sg.small_scene(sg.BLINN, shadow=True, seed=2, width=65, height=17),
+            sg.small_scene(sg.NORMALMAP, shadow=True, seed=7, cam=(0.9, -0.4, 0.6), width=128, height=96),
+            sg.small_scene(sg.TEXTURE_LIGHT, shadow=False, seed=8, width=200, height=37),
+            sg.config5_scene(layers=8, grid=180, width=640, height=360, shadow_size=512, tex_size=256)]
+
+
+@pytest.mark.parametrize("tile,wgs", TUNINGS)
+def test_tunings_match_oracle(oracle, tile, wgs):
+    """Every tuning renders the oracle's bits: separate draws (depth, winners,
+    shadow maps) and DrawModel::draw frames, whose main pass waits inside the
+    persistent raster for its own shadow pass."""
+    from hana_amd.gpu import Context, SceneOnDevice
+    with Context(0) as ctx:
+        ctx.set_tuning(tile, wgs)
+        for scene in frame_scenes():
+            check_scene(ctx, oracle, scene)
+            dev = SceneOnDevice(ctx, scene)
+            dev.render()
+            col, dep = dev.read()
+            tgt, _, _ = oracle.render_frame(scene)
+            assert_depth_exact(dep, tgt.depth, f"{scene.name} frame, tiles {tile}, wgs {wgs}")
+            assert_color_ok(col, tgt.color, f"{scene.name} frame, tiles {tile}, wgs {wgs}")
+
+
+def test_tunings_nan_depth(oracle):
+    """NaN-depth fragments resolve the same way with 32x32 tiles in the persistent raster."""
+    from hana_amd.gpu import Context
+    with Context(0) as ctx:
+        ctx.set_tuning(32, 4)
+        check_scene(ctx, oracle, nan_depth_scene())
+
+
+def test_tuning_rejects_bad_values():
+    from hana_amd.gpu import Context, HanaError
+    with Context(0) as ctx:
+        for bad in [(8, 0, 4), (16, -1, 4), (16, 17, 4), (16, 0, 0), (16, 0, 5)]:
+            with pytest.raises(HanaError):
+                ctx.set_tuning(*bad)
+
+
+def fp32(scene):
+    scene.shadow_fp32 = True
+    scene.name += " fp32 shadow"
+    return scene
+
+
+@pytest.mark.parametrize("tile,wgs", [(16, 0), (32, 4)])
+def test_fp32_shadow_extension(oracle, tile, wgs):
+    """The fp32 shadow-map extension (depth-only shadow pass, the main pass
+    comparing against the map's fp32 depth) against the oracle extended the
+    same way: shadow-map depth and winners bit-exact, its colour untouched,
+    the main pass at the usual bar; per-call draws and whole frames."""
+    from hana_amd.gpu import Context, SceneOnDevice
+    scenes = [fp32(sg.small_scene(sg.NORMALMAP, shadow=True, seed=1)),
+              fp32(sg.small_scene(sg.BLINN, shadow=True, seed=2, width=97, height=61)),
+              fp32(sg.config5_scene(layers=8, grid=180, width=640, height=360, shadow_size=512, tex_size=256))]
+    with Context(0) as ctx:
+        ctx.set_tuning(tile, wgs)
+        for scene in scenes:
+            check_scene(ctx, oracle, scene)
+            dev = SceneOnDevice(ctx, scene)
+            dev.render()
+            col, dep = dev.read()
+            tgt, _, _ = oracle.render_frame(scene)
+            assert_depth_exact(dep, tgt.depth, scene.name + " frame")
+            assert_color_ok(col, tgt.color, scene.name + " frame")

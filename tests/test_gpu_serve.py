@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from hana_amd import scenegen as sg
-from parity import assert_color_ok
+from parity import assert_color_ok, assert_depth_exact
 from pyoracle import CPURenderer
 
 pytestmark = pytest.mark.gpu
@@ -55,7 +55,8 @@ def collect(server, count, timeout_ms=60_000):
             waited += 50
         for c in got:
             img = server.image(c) if c.status == 0 else None
-            out[int(c.user_tag)] = (c.status, img, c.batch_frames)
+            dep = server.depth(c) if c.status == 0 else None
+            out[int(c.user_tag)] = (c.status, img, c.batch_frames, dep)
             server.release(int(c.id))
     return out
 
@@ -92,7 +93,7 @@ def test_same_alone_and_in_the_mix(ctx, workload, resident):
 def test_matches_oracle(ctx, workload, resident):
     from hana_amd.gpu import Server
     reqs = workload.requests[:6]
-    server = Server(ctx, lanes=3, max_batch_frames=4)
+    server = Server(ctx, lanes=3, max_batch_frames=4, return_depth=True)
     submit_all(server, workload, resident, reqs)
     res = collect(server, len(reqs))
     server.close()
@@ -100,6 +101,44 @@ def test_matches_oracle(ctx, workload, resident):
     for i, r in enumerate(reqs):
         tgt, _, _ = o.render_frame(sg.request_scene(workload, r))
         assert_color_ok(res[i][1], tgt.color, f"request {i}")
+        assert_depth_exact(res[i][3], tgt.depth, f"request {i}")
+
+
+def test_config4_full_size_matches_oracle(ctx):
+    """The serving workload at its own sizes: the first 64 requests of the
+    Poisson stream over the 64 resident meshes (5k-500k triangles), 256^2 to
+    1920x1080, Blinn/NormalMap, half shadowed, submitted together and batched
+    across lanes; every completion's colour and depth against the oracle."""
+    from hana_amd.gpu import Server
+    wl = sg.config4_workload(seconds=1.0, max_requests=64)
+    assert len(wl.meshes) == 64 and len(wl.requests) == 64
+    meshes = [ctx.mesh(m) for m in wl.meshes]
+    texs = [(ctx.texture(d), ctx.texture(n)) for d, n in wl.textures]
+    server = Server(ctx, lanes=3, max_batch_frames=16, return_depth=True)
+    for i, r in enumerate(wl.requests):
+        d, n = texs[r["tex"]]
+        server.submit(meshes[r["mesh"]], r["shader"], r["shadow"], r["width"], r["height"], r["cam_pos"],
+                      r["light_pos"], d, n, tag=i)
+    got = {}
+    waited = 0
+    while len(got) < len(wl.requests) and waited < 120_000:
+        cs = server.poll(64, wait_ms=50)
+        waited += 0 if cs else 50
+        for c in cs:
+            got[int(c.user_tag)] = (c.status, server.image(c) if c.status == 0 else None,
+                                    server.depth(c) if c.status == 0 else None, c.batch_frames)
+            server.release(int(c.id))
+    st = server.stats()
+    server.close()
+    assert len(got) == len(wl.requests) and all(v[0] == 0 for v in got.values())
+    assert max(v[3] for v in got.values()) > 1          # batched across requests
+    o = CPURenderer("oracle")
+    for i, r in enumerate(wl.requests):
+        sc = sg.request_scene(wl, r)
+        tgt, _, _ = o.render_frame(sc)
+        assert_depth_exact(got[i][2], tgt.depth, f"request {i} ({sc.width}x{sc.height})")
+        assert_color_ok(got[i][1], tgt.color, f"request {i} ({sc.width}x{sc.height})")
+    assert st["completed"] == len(wl.requests)
 
 
 def test_budget_throttles_admission(ctx, workload, resident):

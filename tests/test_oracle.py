@@ -336,3 +336,82 @@ def test_depth_order_semantics(oracle):
 def test_uniforms_struct_size():
     import ctypes
     assert ctypes.sizeof(Uniforms) == 6 * 64 + 24 + 64 + 16
+
+
+# ------------------------------------------------ the fp32 shadow-map extension
+# No reference counterpart.  Its two changes are pinned against the oracle's
+# parity path, itself pinned to the reference above:
+#  - the shadow pass runs the same depth test and writes the same depth, and
+#    no colour;
+#  - is_in_shadow compares against 2*d - 1 from the map's depth: a map of
+#    depth 1.0 (closest 1.0) must shade exactly as an R8 map of R = 255
+#    (1.0 too), one of depth 0.5 as R = 0, and one of the clear depth
+#    FLT_MAX (closest +inf) exactly as no shadow map at all.
+
+def fp32_scene(seed=11, kind=sg.NORMALMAP):
+    s = sg.small_scene(kind, shadow=True, seed=seed, width=120, height=90)
+    s.shadow_fp32 = True
+    return s
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_fp32_shadow_pass_is_depth_only(oracle, seed):
+    s = fp32_scene(seed)
+    u8 = oracle.build_uniforms(s)
+    u8.shadow_fp32 = 0
+    u32 = oracle.build_uniforms(s)
+    assert u32.shadow_fp32 == 1
+    a, b = HostTarget(120, 90), HostTarget(120, 90)
+    st_a, _ = oracle.draw(s.mesh, sg.SHADOW, u8, a, s.diffuse, s.normal)
+    st_b, _ = oracle.draw(s.mesh, sg.SHADOW, u32, b, s.diffuse, s.normal)
+    assert st_a["fragments"] == st_b["fragments"] > 0
+    assert beq(a.depth, b.depth)
+    assert (b.color[..., :3] == 0).all()                       # untouched clear colour
+    assert (a.color[..., :3] != 0).any()
+
+
+def constant_map(w, h, r=None, depth=None):
+    m = HostTarget(w, h)
+    if r is not None:
+        m.color[..., 0] = r
+    if depth is not None:
+        m.depth[...] = np.float32(depth)
+    return m
+
+
+@pytest.mark.parametrize("kind", [sg.BLINN, sg.NORMALMAP])
+@pytest.mark.parametrize("r,depth", [(255, 1.0), (0, 0.5)])
+def test_fp32_compare_matches_r8_on_equal_maps(oracle, kind, r, depth):
+    s = fp32_scene(13, kind)
+    u8 = oracle.build_uniforms(s)
+    u8.shadow_fp32 = 0
+    u32 = oracle.build_uniforms(s)
+    t8, t32 = HostTarget(120, 90), HostTarget(120, 90)
+    oracle.draw(s.mesh, kind, u8, t8, s.diffuse, s.normal, constant_map(120, 90, r=r))
+    oracle.draw(s.mesh, kind, u32, t32, s.diffuse, s.normal, constant_map(120, 90, depth=depth))
+    assert beq(t8.depth, t32.depth)
+    assert np.array_equal(t8.color, t32.color)
+
+
+def test_fp32_compare_matches_the_reference_on_equal_maps(oracle, ref):
+    """The same equivalence with the compiled reference on the R8 side."""
+    s = fp32_scene(14, sg.NORMALMAP)
+    u8 = ref.build_uniforms(s)
+    u8.shadow_fp32 = 0
+    u32 = oracle.build_uniforms(s)
+    t8, t32 = HostTarget(120, 90), HostTarget(120, 90)
+    ref.draw(s.mesh, sg.NORMALMAP, u8, t8, s.diffuse, s.normal, constant_map(120, 90, r=255))
+    oracle.draw(s.mesh, sg.NORMALMAP, u32, t32, s.diffuse, s.normal, constant_map(120, 90, depth=1.0))
+    assert beq(t8.depth, t32.depth)
+    assert np.array_equal(t8.color[..., :3], t32.color[..., :3])
+
+
+def test_fp32_uncovered_map_lights_everything(oracle):
+    s = fp32_scene(15, sg.BLINN)
+    u32 = oracle.build_uniforms(s)
+    u0 = oracle.build_uniforms(s, enable_shadow=False)
+    t32, t0 = HostTarget(120, 90), HostTarget(120, 90)
+    oracle.draw(s.mesh, sg.BLINN, u32, t32, s.diffuse, s.normal, constant_map(120, 90))   # depth FLT_MAX
+    oracle.draw(s.mesh, sg.BLINN, u0, t0, s.diffuse, s.normal, None)
+    assert beq(t32.depth, t0.depth)
+    assert np.array_equal(t32.color, t0.color)
