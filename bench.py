@@ -726,7 +726,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--sections", default="c1,c2,c3,c4,c5")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tuning", default="", help="batch tuning tile,wgs_per_cu,parts instead of the autotuner")
+    ap.add_argument("--tuning", default="", help="batch tuning tile,wgs_per_cu,parts[,waves] instead of the autotuner")
     ap.add_argument("--c3-batch-only", action="store_true", help="the batch alone (profiling runs)")
     ap.add_argument("--skip-dropin", action="store_true")
     ap.add_argument("--batch-reps", type=int, default=3)

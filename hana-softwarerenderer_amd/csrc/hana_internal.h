@@ -56,6 +56,7 @@ struct PassSpec {
 struct Tuning {
     int tile_shift = TILE_SHIFT_DEFAULT;   // tiles of 2^shift pixels a side
     int raster_wgs_per_cu = 0;            // 0: one raster workgroup per tile; else a persistent raster
+    int raster_waves = 4;                  // waves per raster workgroup (2 or 4)
     int parts = 4;                         // concurrent parts of a batch
 };
 
@@ -83,6 +84,7 @@ struct Plan {
     int total_tiles = 0;
     bool small_tris = true;   // every pass has at most 4 target pixels per face
     int tile_shift = TILE_SHIFT_DEFAULT;
+    int raster_waves = 4;
     int raster_wgs = 0;       // workgroups of the persistent raster (0: one per tile and group)
     uint32_t q_g0 = 0, q_dep_from = 0;
     Workspace ws{};

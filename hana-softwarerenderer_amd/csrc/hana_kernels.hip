@@ -946,15 +946,18 @@ constexpr int SMALL_PIX = HANA_SMALL_PIX;   // triangles with at most this many 
 __device__ __forceinline__ uint32_t tile_div_magic(int w) { return (65536u + (uint32_t)w - 1u) / (uint32_t)w; }
 __device__ __forceinline__ int tile_div(int k, uint32_t magic) { return (int)(((uint32_t)k * magic) >> 16); }
 
-template <int TS>
+// The LDS of one raster workgroup: RB lanes (2 or 4 waves) over a tile of
+// 2^TS pixels a side.
+template <int TS, int RB>
 struct TileLds {
     static constexpr int PIX = 1 << (2 * TS);
+    static constexpr int LANES = RB;
     uint64_t key[PIX];
     uint32_t nan_order[PIX];   // 1 + order of the last NaN-depth fragment
-    float ax[RBLOCK], ay[RBLOCK], s0x[RBLOCK], s0y[RBLOCK], s1x[RBLOCK], s1y[RBLOCK], uz[RBLOCK];
-    float z0[RBLOCK], z1[RBLOCK], z2[RBLOCK];
-    uint32_t lo[RBLOCK], box[RBLOCK];
-    int big[RBLOCK];
+    float ax[RB], ay[RB], s0x[RB], s0y[RB], s1x[RB], s1y[RB], uz[RB];
+    float z0[RB], z1[RB], z2[RB];
+    uint32_t lo[RB], box[RB];
+    int big[RB];
     int nbig, head, nan_seen;
 };
 
@@ -975,8 +978,8 @@ __device__ __forceinline__ bool load_tri(const PassDesc& P, const Workspace& ws,
 }
 
 // One fragment of triangle t at tile-local pixel (lx, ly).
-template <int TS>
-__device__ __forceinline__ void frag_resolve(TileLds<TS>& L, const TriS& t, uint32_t lo, int xb, int yb, int lx, int ly,
+template <int TS, int RB>
+__device__ __forceinline__ void frag_resolve(TileLds<TS, RB>& L, const TriS& t, uint32_t lo, int xb, int yb, int lx, int ly,
                                              bool second) {
     float wx, wy, wz, z;
     if (!tri_cover(t, xb + lx, yb + ly, wx, wy, wz, z)) return;
@@ -999,13 +1002,13 @@ __device__ __forceinline__ void frag_resolve(TileLds<TS>& L, const TriS& t, uint
 
 // One sweep over the tile's bin: small triangles one per lane, larger ones
 // one per wave with the lanes spread over their candidate pixels.
-template <int TS>
-__device__ void sweep(TileLds<TS>& L, const PassDesc& P, const Workspace& ws, uint32_t beg, uint32_t cnt, int xb, int yb,
+template <int TS, int RB>
+__device__ void sweep(TileLds<TS, RB>& L, const PassDesc& P, const Workspace& ws, uint32_t beg, uint32_t cnt, int xb, int yb,
                       bool second) {
     constexpr int EDGE = 1 << TS;
     int tid = threadIdx.x, lane = tid & 63;
     bool er = cnt > (uint32_t)HANA_ER_MIN_ENTRIES;   // uniform over the workgroup
-    for (uint32_t base = 0; base < cnt; base += RBLOCK) {
+    for (uint32_t base = 0; base < cnt; base += RB) {
         bool ok = false;
         TriS t;
         uint32_t order = 0;
@@ -1033,11 +1036,11 @@ __device__ void sweep(TileLds<TS>& L, const PassDesc& P, const Workspace& ws, ui
                         int b = __builtin_ctz(mask);
                         mask &= mask - 1u;
                         int row = tile_div(b, magic);
-                        frag_resolve<TS>(L, t, lo, xb, yb, bx0 + b - row * wdt, by0 + row, second);
+                        frag_resolve<TS, RB>(L, t, lo, xb, yb, bx0 + b - row * wdt, by0 + row, second);
                     }
                 } else
                     for (int ly = by0; ly <= by1; ly++)
-                        for (int lx = bx0; lx <= bx1; lx++) frag_resolve<TS>(L, t, lo, xb, yb, lx, ly, second);
+                        for (int lx = bx0; lx <= bx1; lx++) frag_resolve<TS, RB>(L, t, lo, xb, yb, lx, ly, second);
             } else {
                 L.ax[tid] = t.ax; L.ay[tid] = t.ay; L.s0x[tid] = t.s0x; L.s0y[tid] = t.s0y;
                 L.s1x[tid] = t.s1x; L.s1y[tid] = t.s1y; L.uz[tid] = t.uz;
@@ -1066,7 +1069,7 @@ __device__ void sweep(TileLds<TS>& L, const PassDesc& P, const Workspace& ws, ui
             uint32_t magic = tile_div_magic(wdt);
             for (int k = lane; k < npix; k += 64) {
                 int row = tile_div(k, magic);
-                frag_resolve<TS>(L, bt, blo, xb, yb, x0 + k - row * wdt, y0 + row, second);
+                frag_resolve<TS, RB>(L, bt, blo, xb, yb, x0 + k - row * wdt, y0 + row, second);
             }
         }
         __syncthreads();
@@ -1153,8 +1156,8 @@ __device__ __forceinline__ int xcd_logical_block(int b, int nb) {
 // fragment program per covered pixel; a tile no triangle reaches only takes
 // the pass's clear.  The workgroup's threads all call it; it leaves L in use
 // until they have all returned.
-template <int TS>
-__device__ void raster_tile(TileLds<TS>& L, const PassDesc& P, const Workspace& ws, int lt) {
+template <int TS, int RB>
+__device__ void raster_tile(TileLds<TS, RB>& L, const PassDesc& P, const Workspace& ws, int lt) {
     constexpr int EDGE = 1 << TS, PIX = 1 << (2 * TS);
     int tx = lt % P.tiles_x, ty = lt / P.tiles_x;
     int gt = P.tile0 + lt;
@@ -1167,7 +1170,7 @@ __device__ void raster_tile(TileLds<TS>& L, const PassDesc& P, const Workspace& 
     if (beg >= ws.bin_cap) cnt = 0;
     else if (cnt > ws.bin_cap - beg) cnt = ws.bin_cap - beg;
     if (cnt == 0) {   // uniform: no fragment, the clear only
-        for (int i = tid; i < PIX; i += RBLOCK) {
+        for (int i = tid; i < PIX; i += RB) {
             int px = xb + (i & (EDGE - 1)), py = yb + (i >> TS);
             if (px >= W || py >= H) continue;
             size_t pix = (size_t)py * W + px;
@@ -1180,7 +1183,7 @@ __device__ void raster_tile(TileLds<TS>& L, const PassDesc& P, const Workspace& 
         return;
     }
 
-    for (int i = tid; i < PIX; i += RBLOCK) {
+    for (int i = tid; i < PIX; i += RB) {
         int px = xb + (i & (EDGE - 1)), py = yb + (i >> TS);
         uint64_t k = ~0ull;
         if (px < W && py < H) k = init_key(P.clear ? P.clear_depth : P.depth[(size_t)py * W + px]);
@@ -1190,20 +1193,20 @@ __device__ void raster_tile(TileLds<TS>& L, const PassDesc& P, const Workspace& 
     if (tid == 0) { L.nbig = 0; L.head = 0; L.nan_seen = 0; }
     __syncthreads();
 
-    sweep<TS>(L, P, ws, beg, cnt, xb, yb, false);
+    sweep<TS, RB>(L, P, ws, beg, cnt, xb, yb, false);
 
     if (L.nan_seen) {
         // pixels that saw a NaN-depth fragment restart after the last one
-        for (int i = tid; i < PIX; i += RBLOCK) {
+        for (int i = tid; i < PIX; i += RB) {
             uint32_t no = L.nan_order[i];
             if (no) L.key[i] = (0xFFFFFFFFull << 32) | (uint64_t)(0xFFFFFFFEu - (no - 1u));
         }
         __syncthreads();
-        sweep<TS>(L, P, ws, beg, cnt, xb, yb, true);
+        sweep<TS, RB>(L, P, ws, beg, cnt, xb, yb, true);
     }
 
     uint32_t written = 0;
-    for (int i = tid; i < PIX; i += RBLOCK) {
+    for (int i = tid; i < PIX; i += RB) {
         int px = xb + (i & (EDGE - 1)), py = yb + (i >> TS);
         if (px >= W || py >= H) continue;
         uint32_t lo = (uint32_t)L.key[i];
@@ -1225,14 +1228,14 @@ __device__ void raster_tile(TileLds<TS>& L, const PassDesc& P, const Workspace& 
 // One workgroup per tile of one dependency group (passes of the group, their
 // tiles numbered from 0 by tblock0), XCD-banded.  GROUP only names the launch
 // (the shadow passes' group 0, the main passes' group 1) in profiles.
-template <int TS, int GROUP>
-__global__ __launch_bounds__(RBLOCK) void k_raster(const PassDesc* passes, int npasses, Workspace ws) {
-    __shared__ TileLds<TS> L;
+template <int TS, int GROUP, int RB>
+__global__ __launch_bounds__(RB) void k_raster(const PassDesc* passes, int npasses, Workspace ws) {
+    __shared__ TileLds<TS, RB> L;
     // an incomplete sequence leaves its targets untouched: the host renders it again
     if (ws.status[STATUS_SEQ_FLAGS] & FLAG_NO_RASTER) return;
     int blk = xcd_logical_block(blockIdx.x, gridDim.x);
     const PassDesc& P = passes[find_pass<2>(passes, npasses, blk)];
-    raster_tile<TS>(L, P, ws, blk - P.tblock0);
+    raster_tile<TS, RB>(L, P, ws, blk - P.tblock0);
 }
 
 // The persistent raster: a fixed grid of workgroups takes the tiles of every
@@ -1265,9 +1268,9 @@ __device__ __forceinline__ int pass_of_item(const PassDesc* P, int n, uint32_t i
     return lo;
 }
 
-template <int TS>
-__global__ __launch_bounds__(RBLOCK) void k_raster_q(const PassDesc* passes, int npasses, Workspace ws) {
-    __shared__ TileLds<TS> L;
+template <int TS, int RB>
+__global__ __launch_bounds__(RB) void k_raster_q(const PassDesc* passes, int npasses, Workspace ws) {
+    __shared__ TileLds<TS, RB> L;
     __shared__ uint32_t s_first;
     if (ws.status[STATUS_SEQ_FLAGS] & FLAG_NO_RASTER) return;
     const int tid = threadIdx.x;
@@ -1307,7 +1310,7 @@ __global__ __launch_bounds__(RBLOCK) void k_raster_q(const PassDesc* passes, int
                 ready = true;
             }
             const PassDesc& P = passes[pass_of_item(passes, npasses, it)];
-            raster_tile<TS>(L, P, ws, (int)(it - (uint32_t)P.tile0));
+            raster_tile<TS, RB>(L, P, ws, (int)(it - (uint32_t)P.tile0));
             __syncthreads();   // the tile's writes are issued and L is free again
             if (it < ws.q_g0) mine++;
         }
@@ -1384,25 +1387,36 @@ hipError_t launch_bin(const PassDesc* d_passes, const Workspace& ws, int nseg, b
         hipLaunchKernelGGL(k_bin<11>, dim3(nseg + 512), dim3(BLOCK), 0, s, d_passes, ws, nseg);
     return hipGetLastError();
 }
+template <int TS, int RB>
+static void raster_launch(const PassDesc* d_passes, int npasses, const Workspace& ws, int blocks, int group,
+                          hipStream_t s) {
+    if (group) hipLaunchKernelGGL((k_raster<TS, 1, RB>), dim3(blocks), dim3(RB), 0, s, d_passes, npasses, ws);
+    else hipLaunchKernelGGL((k_raster<TS, 0, RB>), dim3(blocks), dim3(RB), 0, s, d_passes, npasses, ws);
+}
 hipError_t launch_raster(const PassDesc* d_passes, int npasses, const Workspace& ws, int blocks, int group,
                          hipStream_t s) {
     if (blocks <= 0) return hipSuccess;
+    bool two = ws.raster_waves == 2;
     if (ws.tile_shift == 5) {
-        if (group) hipLaunchKernelGGL((k_raster<5, 1>), dim3(blocks), dim3(RBLOCK), 0, s, d_passes, npasses, ws);
-        else hipLaunchKernelGGL((k_raster<5, 0>), dim3(blocks), dim3(RBLOCK), 0, s, d_passes, npasses, ws);
+        if (two) raster_launch<5, 128>(d_passes, npasses, ws, blocks, group, s);
+        else raster_launch<5, 256>(d_passes, npasses, ws, blocks, group, s);
     } else {
-        if (group) hipLaunchKernelGGL((k_raster<4, 1>), dim3(blocks), dim3(RBLOCK), 0, s, d_passes, npasses, ws);
-        else hipLaunchKernelGGL((k_raster<4, 0>), dim3(blocks), dim3(RBLOCK), 0, s, d_passes, npasses, ws);
+        if (two) raster_launch<4, 128>(d_passes, npasses, ws, blocks, group, s);
+        else raster_launch<4, 256>(d_passes, npasses, ws, blocks, group, s);
     }
     return hipGetLastError();
 }
 hipError_t launch_raster_queue(const PassDesc* d_passes, int npasses, const Workspace& ws, int workgroups,
                                hipStream_t s) {
     if (workgroups <= 0 || ws.q_items == 0) return hipSuccess;
-    if (ws.tile_shift == 5)
-        hipLaunchKernelGGL(k_raster_q<5>, dim3(workgroups), dim3(RBLOCK), 0, s, d_passes, npasses, ws);
-    else
-        hipLaunchKernelGGL(k_raster_q<4>, dim3(workgroups), dim3(RBLOCK), 0, s, d_passes, npasses, ws);
+    bool two = ws.raster_waves == 2;
+    if (ws.tile_shift == 5) {
+        if (two) hipLaunchKernelGGL((k_raster_q<5, 128>), dim3(workgroups), dim3(128), 0, s, d_passes, npasses, ws);
+        else hipLaunchKernelGGL((k_raster_q<5, 256>), dim3(workgroups), dim3(256), 0, s, d_passes, npasses, ws);
+    } else {
+        if (two) hipLaunchKernelGGL((k_raster_q<4, 128>), dim3(workgroups), dim3(128), 0, s, d_passes, npasses, ws);
+        else hipLaunchKernelGGL((k_raster_q<4, 256>), dim3(workgroups), dim3(256), 0, s, d_passes, npasses, ws);
+    }
     return hipGetLastError();
 }
 hipError_t launch_present(const uint32_t* src, uint32_t* dst, int w, int h, hipStream_t s) {

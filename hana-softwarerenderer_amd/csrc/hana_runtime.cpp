@@ -212,6 +212,7 @@ int build_plan(hana_ctx* c, std::vector<PassSpec> specs, Plan& pl, const Tuning&
     pl.total_f = fo;
     pl.total_tiles = tile0;
     pl.tile_shift = tshift;
+    pl.raster_waves = tu.raster_waves;
     pl.raster_wgs = tu.raster_wgs_per_cu > 0 ? tu.raster_wgs_per_cu * c->num_cus : 0;
     pl.q_g0 = (uint32_t)tb[0];
     pl.q_dep_from = pl.q_dep_from < (uint32_t)n ? (uint32_t)pl.descs[pl.q_dep_from].tile0 : (uint32_t)tile0;
@@ -333,6 +334,7 @@ int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_p
     ws.q_g0 = pl.q_g0;
     ws.q_dep_from = pl.q_dep_from;
     ws.tile_shift = pl.tile_shift;
+    ws.raster_waves = pl.raster_waves;
     return HANA_OK;
 }
 
@@ -679,10 +681,11 @@ bool valid_kind(int k) { return k >= HANA_SHADER_GROUND && k <= HANA_SHADER_SHAD
 
 int tuning_from(hana_ctx* c, const hana_tuning* t, Tuning& out) {
     if (!t || !(t->tile_size == 16 || t->tile_size == 32) || t->raster_wgs_per_cu < 0 || t->raster_wgs_per_cu > 16 ||
-        t->parts < 1 || t->parts > (int)kMaxBatchParts)
-        return fail(c, HANA_E_INVALID, "tuning: tile_size 16 or 32, raster_wgs_per_cu 0-16, parts 1-4");
+        t->parts < 1 || t->parts > (int)kMaxBatchParts || !(t->raster_waves == 0 || t->raster_waves == 2 || t->raster_waves == 4))
+        return fail(c, HANA_E_INVALID, "tuning: tile_size 16 or 32, raster_wgs_per_cu 0-16, parts 1-4, raster_waves 0, 2 or 4");
     out.tile_shift = t->tile_size == 32 ? 5 : 4;
     out.raster_wgs_per_cu = t->raster_wgs_per_cu;
+    out.raster_waves = t->raster_waves ? t->raster_waves : 4;
     out.parts = t->parts;
     return HANA_OK;
 }
@@ -1345,6 +1348,7 @@ static void tuning_out(const Tuning& t, double ms, hana_tuning* out) {
     out->tile_size = tile_edge(t.tile_shift);
     out->raster_wgs_per_cu = t.raster_wgs_per_cu;
     out->parts = t.parts;
+    out->raster_waves = t.raster_waves;
     out->ms = ms;
 }
 
@@ -1371,8 +1375,9 @@ int hana_batch_get_tuning(hana_ctx* c, const hana_batch* b, hana_tuning* out) {
 
 // The candidates, one dimension at a time from the batch's current tuning:
 // the tile size, then the raster form (per-tile workgroups or a persistent
-// grid of 2, 4 or 8 workgroups per CU), then the number of parts; each keeps
-// the best value found so far for the others.
+// grid of 2, 4 or 8 workgroups per CU), then the waves per raster workgroup,
+// then the number of parts; each keeps the best value found so far for the
+// others.
 int hana_batch_autotune(hana_ctx* c, hana_batch* b, int use_graph, int reps, hana_tuning* chosen) {
     if (!c || !b || reps < 1) return HANA_E_INVALID;
     int r;
@@ -1386,15 +1391,19 @@ int hana_batch_autotune(hana_ctx* c, hana_batch* b, int use_graph, int reps, han
         return e ? e : batch_time(c, b, reps, ms);
     };
     if ((r = measure(best, &best_ms))) return r;
-    const int shifts[] = {4, 5}, wgs[] = {0, 2, 4, 8}, parts[] = {1, 2, 4};
-    for (int dim = 0; dim < 3; dim++) {
-        const int* vals = dim == 0 ? shifts : (dim == 1 ? wgs : parts);
-        int nv = dim == 0 ? 2 : (dim == 1 ? 4 : 3);
+    const int shifts[] = {4, 5}, wgs[] = {0, 2, 4, 8}, waves[] = {4, 2}, parts[] = {1, 2, 4};
+    const int* dims[4] = {shifts, wgs, waves, parts};
+    const int counts[4] = {2, 4, 2, 3};
+    for (int dim = 0; dim < 4; dim++) {
+        const int* vals = dims[dim];
+        int nv = counts[dim];
         Tuning base = best;
         for (int i = 0; i < nv; i++) {
             Tuning t = base;
-            int& field = dim == 0 ? t.tile_shift : (dim == 1 ? t.raster_wgs_per_cu : t.parts);
-            const int cur = dim == 0 ? base.tile_shift : (dim == 1 ? base.raster_wgs_per_cu : base.parts);
+            int* fields[4] = {&t.tile_shift, &t.raster_wgs_per_cu, &t.raster_waves, &t.parts};
+            const int curs[4] = {base.tile_shift, base.raster_wgs_per_cu, base.raster_waves, base.parts};
+            int& field = *fields[dim];
+            const int cur = curs[dim];
             if (vals[i] == cur) continue;
             field = vals[i];
             double ms = 0.0;

@@ -22,9 +22,6 @@ constexpr int TILE_SHIFT_DEFAULT = 4;
 constexpr int TILE_SHIFT_MAX = 5;
 __host__ __device__ constexpr int tile_edge(int shift) { return 1 << shift; }
 constexpr int BLOCK = 256;                 // threads per workgroup (4 wave64)
-#ifndef HANA_RASTER_BLOCK
-#define HANA_RASTER_BLOCK 256
-#endif
 #ifndef HANA_SMALL_PIX
 #define HANA_SMALL_PIX 32
 #endif
@@ -38,8 +35,8 @@ constexpr int BLOCK = 256;                 // threads per workgroup (4 wave64)
 #ifndef HANA_ER_MIN_ENTRIES
 #define HANA_ER_MIN_ENTRIES 320
 #endif
-constexpr int RBLOCK = HANA_RASTER_BLOCK;  // threads per raster workgroup (one tile)
-static_assert(RBLOCK % 64 == 0 && RBLOCK <= 256, "raster workgroup: whole waves, at most one lane per pixel of a 16x16 tile");
+// A raster workgroup (one tile at a time) has 2 or 4 waves: 128 or 256 lanes,
+// at most one lane per pixel of a 16x16 tile.
 static_assert(HANA_SMALL_PIX >= 1 && HANA_SMALL_PIX <= 32, "small-triangle candidates fit a 32-bit mask");
 #ifndef HANA_VPT
 #define HANA_VPT 4
@@ -150,6 +147,7 @@ struct Workspace {
     uint32_t q_g0;               // tiles of group-0 passes
     uint32_t q_dep_from;         // first item of a pass that samples a map drawn by group 0
     int32_t tile_shift;          // tiles are 2^tile_shift pixels a side
+    int32_t raster_waves;        // waves per raster workgroup: 2 or 4
 };
 
 // the tile scan: SCAN_CHUNK tile counters per workgroup of SCAN_THREADS

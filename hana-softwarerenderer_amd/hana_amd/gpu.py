@@ -75,11 +75,11 @@ class Completion(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("tile_size", C.c_int32), ("raster_wgs_per_cu", C.c_int32), ("parts", C.c_int32),
-                ("pad_", C.c_int32), ("ms", C.c_double)]
+                ("raster_waves", C.c_int32), ("ms", C.c_double)]
 
     def as_dict(self):
         return {"tile_size": int(self.tile_size), "raster_wgs_per_cu": int(self.raster_wgs_per_cu),
-                "parts": int(self.parts), "ms": float(self.ms)}
+                "parts": int(self.parts), "raster_waves": int(self.raster_waves), "ms": float(self.ms)}
 
 
 class ServerStats(C.Structure):
@@ -376,9 +376,9 @@ class Context:
 
     STAGES = ("vertex", "setup", "clip", "scan", "bin", "unused", "raster_g0", "raster_g1")
 
-    def set_tuning(self, tile_size=16, raster_wgs_per_cu=0, parts=4):
+    def set_tuning(self, tile_size=16, raster_wgs_per_cu=0, parts=4, raster_waves=4):
         '''Tuning of this context's per-call draws (hana_gpu_set_tuning).'''
-        t = Tuning(int(tile_size), int(raster_wgs_per_cu), int(parts), 0, 0.0)
+        t = Tuning(int(tile_size), int(raster_wgs_per_cu), int(parts), int(raster_waves), 0.0)
         self.check(self.lib.hana_gpu_set_tuning(self.h, C.byref(t)), "set_tuning")
 
     def set_timing(self, on: bool):
@@ -476,8 +476,8 @@ class Batch:
     def finalize(self, use_graph=True):
         self.ctx.check(self.ctx.lib.hana_batch_finalize(self.ctx.h, self.h, int(use_graph)), "batch_finalize")
 
-    def set_tuning(self, tile_size=16, raster_wgs_per_cu=0, parts=4):
-        t = Tuning(int(tile_size), int(raster_wgs_per_cu), int(parts), 0, 0.0)
+    def set_tuning(self, tile_size=16, raster_wgs_per_cu=0, parts=4, raster_waves=4):
+        t = Tuning(int(tile_size), int(raster_wgs_per_cu), int(parts), int(raster_waves), 0.0)
         self.ctx.check(self.ctx.lib.hana_batch_set_tuning(self.ctx.h, self.h, C.byref(t)), "batch_set_tuning")
 
     def autotune(self, use_graph=True, reps=3) -> dict:

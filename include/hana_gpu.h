@@ -200,14 +200,16 @@ int hana_render_frame(hana_ctx* ctx, const hana_frame_desc* f);
  * raster).  raster_wgs_per_cu: 0 rasterises each pass group with one
  * workgroup per tile; n > 0 runs one persistent raster of n workgroups per CU
  * that takes the tiles of every pass from a queue (shadow passes first, main
- * passes waiting for them without a second launch).  parts: the concurrent
- * parts a batch is split into (1-4; ignored for per-call draws).  Results are
- * identical under every tuning.  Default: 16, 0, 4. */
+ * passes waiting for them without a second launch).  raster_waves: lanes per
+ * raster workgroup, 2 or 4 waves (two waves fit twice the tiles on a CU).
+ * parts: the concurrent parts a batch is split into (1-4; ignored for
+ * per-call draws).  Results are identical under every tuning.  Default: 16,
+ * 0, 4 parts, 4 waves. */
 typedef struct hana_tuning {
     int32_t tile_size;
     int32_t raster_wgs_per_cu;
     int32_t parts;
-    int32_t pad_;
+    int32_t raster_waves;           /* waves per raster workgroup: 2 or 4 (0: 4) */
     double ms;                      /* hana_batch_autotune: measured run time of the choice */
 } hana_tuning;
 /* Tuning of the context's per-call draws (hana_draw, hana_render_frame). */
@@ -226,7 +228,8 @@ int hana_batch_set_tuning(hana_ctx* ctx, hana_batch* b, const hana_tuning* t);
 int hana_batch_finalize(hana_ctx* ctx, hana_batch* b, int use_graph);
 /* Finalizes the batch with the tuning that ran it fastest: from the batch's
  * tuning, tile size, then raster form (per tile, or 2, 4 or 8 persistent
- * workgroups per CU), then parts (1, 2, 4), one dimension at a time, each
+ * workgroups per CU), then raster waves (4, 2), then parts (1, 2, 4), one
+ * dimension at a time, each
  * candidate sized, captured and timed over `reps` runs.  The frames are
  * rendered by every run.  `chosen` (nullable) receives the choice and its
  * time. */

@@ -202,18 +202,20 @@ def test_empty_mesh_frames(ctx, empty_first):
         batch.close()
 
 
-@pytest.mark.parametrize("tile,wgs,parts", [(32, 0, 4), (16, 4, 2), (32, 8, 1), (16, 2, 4)])
-def test_batch_tunings(ctx, scenes, oracle_frames, tile, wgs, parts):
-    """Every batch tuning (tile size, persistent raster, parts) renders the
-    oracle's frames, on two graph runs."""
+@pytest.mark.parametrize("tile,wgs,parts,waves", [(32, 0, 4, 4), (16, 4, 2, 4), (32, 8, 1, 2), (16, 2, 4, 4),
+                                                  (16, 0, 4, 2), (16, 4, 4, 2)])
+def test_batch_tunings(ctx, scenes, oracle_frames, tile, wgs, parts, waves):
+    """Every batch tuning (tile size, persistent raster, raster waves, parts)
+    renders the oracle's frames, on two graph runs."""
     from hana_amd.gpu import Batch, SceneOnDevice
     devs = [SceneOnDevice(ctx, s) for s in scenes]
     batch = Batch(ctx)
     for d in devs:
         batch.add(d.desc())
-    batch.set_tuning(tile, wgs, parts)
+    batch.set_tuning(tile, wgs, parts, waves)
     batch.finalize(use_graph=True)
-    assert batch.tuning()["tile_size"] == tile
+    t = batch.tuning()
+    assert (t["tile_size"], t["raster_wgs_per_cu"], t["parts"], t["raster_waves"]) == (tile, wgs, parts, waves)
     for run in range(2):
         poison(ctx, devs)
         batch.run()
@@ -233,7 +235,7 @@ def test_autotune_picks_a_tuning_and_renders_exactly(ctx, scenes, oracle_frames)
         batch.add(d.desc())
     chosen = batch.autotune(use_graph=True, reps=2)
     assert chosen["tile_size"] in (16, 32) and chosen["raster_wgs_per_cu"] in (0, 2, 4, 8)
-    assert chosen["parts"] in (1, 2, 4) and chosen["ms"] > 0
+    assert chosen["parts"] in (1, 2, 4) and chosen["raster_waves"] in (2, 4) and chosen["ms"] > 0
     assert batch.tuning() == chosen
     poison(ctx, devs)
     batch.run()

@@ -502,7 +502,7 @@ def test_clip_pool_overflow_in_a_batch(ctx, oracle):
 
 # Tunings of the per-call path: 32x32 tiles, the persistent raster (both pass
 # groups in one launch, main passes waiting for the shadow tiles), and both.
-TUNINGS = [(32, 0), (16, 4), (32, 8), (16, 1)]
+TUNINGS = [(32, 0, 4), (16, 4, 4), (32, 8, 2), (16, 1, 4), (16, 0, 2), (32, 4, 2)]
 
 
 def frame_scenes():
@@ -513,14 +513,15 @@ def frame_scenes():
             sg.config5_scene(layers=8, grid=180, width=640, height=360, shadow_size=512, tex_size=256)]
 
 
-@pytest.mark.parametrize("tile,wgs", TUNINGS)
-def test_tunings_match_oracle(oracle, tile, wgs):
+@pytest.mark.parametrize("tile,wgs,waves", TUNINGS)
+def test_tunings_match_oracle(oracle, tile, wgs, waves):
     """Every tuning renders the oracle's bits: separate draws (depth, winners,
     shadow maps) and DrawModel::draw frames, whose main pass waits inside the
-    persistent raster for its own shadow pass."""
+    persistent raster for its own shadow pass; 32x32 tiles, 2-wave raster
+    workgroups."""
     from hana_amd.gpu import Context, SceneOnDevice
     with Context(0) as ctx:
-        ctx.set_tuning(tile, wgs)
+        ctx.set_tuning(tile, wgs, raster_waves=waves)
         for scene in frame_scenes():
             check_scene(ctx, oracle, scene)
             dev = SceneOnDevice(ctx, scene)
@@ -535,14 +536,14 @@ def test_tunings_nan_depth(oracle):
     """NaN-depth fragments resolve the same way with 32x32 tiles in the persistent raster."""
     from hana_amd.gpu import Context
     with Context(0) as ctx:
-        ctx.set_tuning(32, 4)
+        ctx.set_tuning(32, 4, raster_waves=2)
         check_scene(ctx, oracle, nan_depth_scene())
 
 
 def test_tuning_rejects_bad_values():
     from hana_amd.gpu import Context, HanaError
     with Context(0) as ctx:
-        for bad in [(8, 0, 4), (16, -1, 4), (16, 17, 4), (16, 0, 0), (16, 0, 5)]:
+        for bad in [(8, 0, 4), (16, -1, 4), (16, 17, 4), (16, 0, 0), (16, 0, 5), (16, 0, 4, 3), (16, 0, 4, 8)]:
             with pytest.raises(HanaError):
                 ctx.set_tuning(*bad)
 
