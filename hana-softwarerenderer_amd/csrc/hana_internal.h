@@ -105,6 +105,10 @@ struct LoggedCall {
     std::vector<Fill> fills;
 };
 
+// Size of the device array that holds the workspace of
+// hana_render_frame_noalloc (one per GPU, for the life of the process).
+constexpr size_t kNoAllocScratchBytes = size_t(128) << 20;
+
 constexpr int kRing = 32;
 // calls kept for replay; past this many a call first checks the status of
 // those before it (a sync)
@@ -173,6 +177,14 @@ struct hana_ctx {
     std::vector<hana::rt::LoggedCall> log;
     bool replay_overflow = true;
     int64_t replays = 0;   // status checks that rendered the logged calls again
+    // hana_render_frame_noalloc: the device address of this GPU's scratch
+    // array, host-mapped status words of its sequences, whether one ran
+    // since the last status check, and the draw slots that must wait for it
+    void* scratch = nullptr;
+    uint32_t* noalloc_status_host = nullptr;
+    uint32_t* noalloc_status_dev = nullptr;
+    bool noalloc_pending = false;
+    uint32_t noalloc_wait_mask = 0;
 };
 
 // A batch's frames are split into up to kMaxBatchParts parts, each a launch
@@ -225,6 +237,9 @@ int check_status(hana_ctx* c);
 int run_passes(hana_ctx* c, std::vector<PassSpec> specs);
 // Validates a hana_tuning into a Tuning.
 int tuning_from(hana_ctx* c, const hana_tuning* t, Tuning& out);
+// Resolves the device address of the no-alloc scratch array of the context's
+// (current) device, once per device (hana_noalloc.cpp).
+int noalloc_init(hana_ctx* c);
 bool valid_kind(int k);
 
 }  // namespace rt

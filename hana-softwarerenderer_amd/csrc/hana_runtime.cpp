@@ -534,7 +534,7 @@ static int replay_call(hana_ctx* c, LoggedCall& L) {
 // - A clipped polygon with more than POLY_MAX vertices was dropped, the rest
 //   drawn (HANA_E_UNSUPPORTED).
 int check_status(hana_ctx* c) {
-    if (!c->status_pending) return HANA_OK;
+    if (!c->status_pending && !c->noalloc_pending) return HANA_OK;
     int r;
     uint32_t seen = 0;
     for (int attempt = 0;; attempt++) {
@@ -589,8 +589,20 @@ int check_status(hana_ctx* c) {
                                              "grown, render those frames again");
         break;
     }
-    if (seen & FLAG_QUEUE_STALL)
+    // no-alloc frames: no workspace to grow, their errors are reported
+    uint32_t na = 0;
+    if (c->noalloc_pending) {
+        na = c->noalloc_status_host[STATUS_FLAGS];
+        std::memset(c->noalloc_status_host, 0, sizeof(uint32_t) * STATUS_WORDS);
+        c->noalloc_pending = false;
+    }
+    if ((seen | na) & FLAG_QUEUE_STALL)
         return fail(c, HANA_E_DEVICE, "the persistent raster's wait for the shadow tiles timed out");
+    if (na & FLAG_CAPACITY)
+        return fail(c, HANA_E_OVERFLOW,
+                    "a hana_render_frame_noalloc frame outgrew the bins or clip pool of the no-alloc scratch and left "
+                    "its targets untouched; render it with hana_render_frame");
+    seen |= na;
     if (seen & FLAG_PAIR_LIMIT)
         return fail(c, HANA_E_NOMEM,
                     "a draw, frame or batch part covered more than 4,294,967,280 (triangle, tile) pairs and was not "
@@ -645,7 +657,12 @@ int run_passes(hana_ctx* c, std::vector<PassSpec> specs) {
     }
     if ((r = enqueue_geometry(c, pl, A, S, true))) return r;
     // the raster waits for the calls in flight on other slots that touch one
-    // of its targets; calls on disjoint targets run concurrently
+    // of its targets; calls on disjoint targets run concurrently.  After a
+    // no-alloc frame (context stream, targets not noted) every slot waits once.
+    if (a != 0 && (c->noalloc_wait_mask & (1u << a))) {
+        HIP_TRY(c, hipStreamWaitEvent(S, c->slot_done[0], 0));
+        c->noalloc_wait_mask &= ~(1u << a);
+    }
     for (int k = 0; k < c->nslots; k++)
         if (k != a && (c->pend[k][a].size() > kMaxPendingTargets || shares_target(c->pend[k][a], touched))) {
             HIP_TRY(c, hipStreamWaitEvent(S, c->slot_done[k], 0));
@@ -719,6 +736,7 @@ int create_context(int device, int nslots, hana_ctx** out) {
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
         c->num_cus = cus;
     auto undo = [c]() {
+        if (c->noalloc_status_host) (void)hipHostFree(c->noalloc_status_host);
         for (int i = 0; i < kRing; i++)
             if (c->ring_event[i]) (void)hipEventDestroy(c->ring_event[i]);
         for (int k = 0; k < c->nslots; k++) {
@@ -742,6 +760,11 @@ int create_context(int device, int nslots, hana_ctx** out) {
         if (hipHostGetDevicePointer((void**)&c->status_dev[k], host, 0) != hipSuccess) return undo();
         if (hipEventCreateWithFlags(&c->slot_done[k], hipEventDisableTiming) != hipSuccess) return undo();
     }
+    if (hipHostMalloc((void**)&c->noalloc_status_host, sizeof(uint32_t) * STATUS_WORDS) != hipSuccess) return undo();
+    std::memset(c->noalloc_status_host, 0, sizeof(uint32_t) * STATUS_WORDS);
+    if (hipHostGetDevicePointer((void**)&c->noalloc_status_dev, c->noalloc_status_host, 0) != hipSuccess ||
+        noalloc_init(c) != HANA_OK)
+        return undo();
     *out = c;
     return HANA_OK;
 }
@@ -778,6 +801,7 @@ int hana_gpu_destroy(hana_ctx* c) {
         if (c->ring_event[i]) (void)hipEventDestroy(c->ring_event[i]);
     }
     if (c->stats_host) (void)hipHostFree(c->stats_host);
+    if (c->noalloc_status_host) (void)hipHostFree(c->noalloc_status_host);
     for (int k = 0; k < c->nslots; k++) {
         if (slot_status(c, k)) (void)hipHostFree(slot_status(c, k));
         if (c->slot_done[k]) (void)hipEventDestroy(c->slot_done[k]);

@@ -125,6 +125,7 @@ DECLS = [
     ("hana_target_present", C.c_int, [_P, _P, C.POINTER(C.c_uint8)]),
     ("hana_draw", C.c_int, [_P, C.POINTER(DrawDesc)]),
     ("hana_render_frame", C.c_int, [_P, C.POINTER(FrameDesc)]),
+    ("hana_render_frame_noalloc", C.c_int, [_P, C.POINTER(FrameDesc)]),
     ("hana_gpu_set_tuning", C.c_int, [_P, C.POINTER(Tuning)]),
     ("hana_batch_create", C.c_int, [_P, C.POINTER(_P)]),
     ("hana_batch_set_tuning", C.c_int, [_P, _P, C.POINTER(Tuning)]),

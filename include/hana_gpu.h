@@ -193,6 +193,19 @@ int hana_target_present(hana_ctx* ctx, hana_target* t, uint8_t* bgra);
 /* ---- drawing ---- */
 int hana_draw(hana_ctx* ctx, const hana_draw_desc* d);
 int hana_render_frame(hana_ctx* ctx, const hana_frame_desc* f);
+/* hana_render_frame for callers that cannot allocate (code under stream
+ * capture, serving loops that must not call hipMalloc); no counterpart in the
+ * reference.  The same frame and result, on the context stream, with the
+ * workspace in a fixed device array of 128 MiB per GPU (held until the
+ * process ends) instead of allocated buffers: no hipMalloc, hipFree or other
+ * allocating call.  A frame that needs more workspace is rejected with
+ * HANA_E_NOMEM before anything runs; one whose bins or clip pool overflow at
+ * run time leaves its targets untouched and the next wait returns
+ * HANA_E_OVERFLOW.  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (two contexts, two threads) race.  Refused
+ * (HANA_E_INVALID) while per-call draws are pending (hana_gpu_sync first) or
+ * with stage timing or statistics on. */
+int hana_render_frame_noalloc(hana_ctx* ctx, const hana_frame_desc* f);
 
 /* ---- tuning: how launch sequences are laid out ----
  * No reference counterpart (the reference scans each triangle's bounding box
