@@ -421,6 +421,31 @@ def bench_c3(ctx, args, scenes, cpu):
         t_default = (time.perf_counter() - t0) / args.steps
         tuning_default = b0.tuning()
         b0.close()
+    # every candidate of the autotuner's space, timed alone (evidence for the choice)
+    sweep = None
+    if args.c3_sweep:
+        sweep = []
+        for tile in (16, 32):
+            for wgs in (0, 2, 4, 8):
+                for waves in (4, 2):
+                    for parts in (1, 2, 4):
+                        bs = Batch(ctx)
+                        for d in devs:
+                            bs.add(d.desc())
+                        bs.set_tuning(tile, wgs, parts, waves)
+                        bs.finalize(use_graph=True)
+                        for _ in range(2):
+                            bs.run()
+                        ctx.sync()
+                        t0 = time.perf_counter()
+                        for _ in range(args.steps):
+                            bs.run()
+                        ctx.sync()
+                        ms_ = (time.perf_counter() - t0) / args.steps * 1e3
+                        bs.close()
+                        sweep.append({"tile_size": tile, "raster_wgs_per_cu": wgs, "raster_waves": waves,
+                                      "parts": parts, "ms_per_batch": round(ms_, 4)})
+                        log(f"c3 sweep: {sweep[-1]}")
     # autotuned (or the tuning given)
     b = Batch(ctx)
     for d in devs:
@@ -489,6 +514,8 @@ def bench_c3(ctx, args, scenes, cpu):
            "stage_ms": {k: round(v, 5) for k, v in stage_ms.items()},
            "batch_alg_bytes": batch_bytes, "batch_achieved_GBps": round(batch_bytes / t_batch / 1e9, 2),
            "roofline": roofline("c3", stage_ms, launches, stage_bytes), "parity": parity}
+    if sweep:
+        out["tuning_sweep"] = sweep
     if frags:
         out["fragments_per_batch"] = frags
         out["Mfrag_s"] = round(frags / t_batch / 1e6, 2)
@@ -728,6 +755,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tuning", default="", help="batch tuning tile,wgs_per_cu,parts[,waves] instead of the autotuner")
     ap.add_argument("--c3-batch-only", action="store_true", help="the batch alone (profiling runs)")
+    ap.add_argument("--c3-sweep", action="store_true", help="time every tuning of the autotuner's space on the batch")
     ap.add_argument("--skip-dropin", action="store_true")
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--frame-steps", type=int, default=20)
