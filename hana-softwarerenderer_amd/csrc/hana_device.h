@@ -16,6 +16,61 @@
 
 namespace hana {
 
+// Address spaces of the loads.  Pointers held in descriptor tables are
+// generic, so plain loads through them compile to FLAT instructions (their
+// completion is counted with the LDS operations'), and loads of uniform data
+// that follow a store in the same kernel must be repeated per lane.
+//  - Uniform blocks never change during a launch: they are read through the
+//    constant address space, so the compiler keeps them in SGPRs.
+//  - Per-lane data (meshes, post-transform vertices, bins, textures, shadow
+//    maps) is read through the global address space.
+#define HANA_CONST_AS __attribute__((address_space(4)))
+#define HANA_GLOBAL_AS __attribute__((address_space(1)))
+typedef const HANA_CONST_AS DevUniforms& Uniforms;
+typedef const HANA_CONST_AS float* UniformRow;
+__device__ __forceinline__ Uniforms uniforms_of(const DevUniforms* p) { return *(const HANA_CONST_AS DevUniforms*)p; }
+
+template <class T>
+__device__ __forceinline__ T gld(const T* p) { return *(const HANA_GLOBAL_AS T*)p; }
+template <class T>
+__device__ __forceinline__ void gst(T* p, T v) { *(HANA_GLOBAL_AS T*)p = v; }
+typedef float f4_vec __attribute__((ext_vector_type(4)));
+typedef float f2_vec __attribute__((ext_vector_type(2)));
+typedef uint32_t u4_vec __attribute__((ext_vector_type(4)));
+typedef uint32_t u2_vec __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float4 gld4(const float4* p) {
+    f4_vec v = *(const HANA_GLOBAL_AS f4_vec*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint4 gld4(const uint4* p) {
+    u4_vec v = *(const HANA_GLOBAL_AS u4_vec*)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint2 gld2(const uint2* p) {
+    u2_vec v = *(const HANA_GLOBAL_AS u2_vec*)p;
+    return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ void gst4(float4* p, float4 v) {
+    f4_vec w = {v.x, v.y, v.z, v.w};
+    *(HANA_GLOBAL_AS f4_vec*)p = w;
+}
+__device__ __forceinline__ void gst4(uint4* p, uint4 v) {
+    u4_vec w = {v.x, v.y, v.z, v.w};
+    *(HANA_GLOBAL_AS u4_vec*)p = w;
+}
+__device__ __forceinline__ void gst2(float2* p, float2 v) {
+    f2_vec w = {v.x, v.y};
+    *(HANA_GLOBAL_AS f2_vec*)p = w;
+}
+__device__ __forceinline__ void gst2(uint2* p, uint2 v) {
+    u2_vec w = {v.x, v.y};
+    *(HANA_GLOBAL_AS u2_vec*)p = w;
+}
+// atomicAdd at agent scope, relaxed, as a global (not FLAT) atomic
+__device__ __forceinline__ uint32_t gadd(uint32_t* p, uint32_t v) {
+    return __hip_atomic_fetch_add((HANA_GLOBAL_AS uint32_t*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 struct F3 { float x, y, z; };
 
 __device__ __forceinline__ F3 f3(float x, float y, float z) { F3 v; v.x = x; v.y = y; v.z = z; return v; }
@@ -31,8 +86,10 @@ __device__ __forceinline__ float dot3(F3 a, F3 b) {
     r += a.x * b.x;
     return r;
 }
-// Matrix row * (x, y, z, w) (SR/matrix.h:112-116)
-__device__ __forceinline__ float dot4(const float* row, float x, float y, float z, float w) {
+// Matrix row * (x, y, z, w) (SR/matrix.h:112-116); the row may live in the
+// constant address space (uniform blocks)
+template <class Row>
+__device__ __forceinline__ float dot4(Row row, float x, float y, float z, float w) {
     float r = 0.0f;
     r += row[3] * w;
     r += row[2] * z;
@@ -72,7 +129,8 @@ __device__ __forceinline__ C3 cmulf(C3 a, float i) {
     return c3(clamp01s(a.r * i), clamp01s(a.g * i), clamp01s(a.b * i));
 }
 __device__ __forceinline__ C3 cmulc(C3 a, C3 b) { return c3(a.r * b.r, a.g * b.g, a.b * b.b); }
-__device__ __forceinline__ C3 c3p(const float* p) { return c3(p[0], p[1], p[2]); }
+template <class Ptr>
+__device__ __forceinline__ C3 c3p(Ptr p) { return c3(p[0], p[1], p[2]); }
 
 // ---- textures: RGBA8 words holding TGAColor bytes b,g,r,a (SR/tgaimage.h:46-53) ----
 struct Tex {
@@ -83,7 +141,7 @@ struct Tex {
 __device__ __forceinline__ uint32_t tex_fetch(const Tex& t, float u, float v) {
     int x = f2i(u * (float)t.w), y = f2i(v * (float)t.h);
     if (!t.texels || x < 0 || y < 0 || x >= t.w || y >= t.h) return 0u;
-    return t.texels[(size_t)y * (size_t)t.w + (size_t)x];
+    return gld(t.texels + ((size_t)y * (size_t)t.w + (size_t)x));
 }
 // Color(TGAColor) (SR/color.cpp:5) for IShader::tex_diffuse (SR/IShader.h:85-89)
 __device__ __forceinline__ C3 tex_diffuse(const Tex& t, float u, float v) {
@@ -116,9 +174,9 @@ __host__ __device__ __forceinline__ int varyings_of(int kind) {
 //   TEXTURE         : uv
 //   GROUND/TOON     : intensity
 //   SHADOW          : clip_pos.z
-__device__ __forceinline__ void vertex_program(int kind, const DevUniforms& u, F3 p, F3 n_obj, float tu, float tv,
+__device__ __forceinline__ void vertex_program(int kind, Uniforms u, F3 p, F3 n_obj, float tu, float tv,
                                                float clip[4], float vary[8]) {
-    const float* m = (kind == SHADER_SHADOW) ? u.light_m : u.clip_m;   // ObjectToViewPos / ObjectToClipPos
+    UniformRow m = (kind == SHADER_SHADOW) ? u.light_m : u.clip_m;   // ObjectToViewPos / ObjectToClipPos
     clip[3] = dot4(m + 12, p.x, p.y, p.z, 1.0f);
     clip[2] = dot4(m + 8, p.x, p.y, p.z, 1.0f);
     clip[1] = dot4(m + 4, p.x, p.y, p.z, 1.0f);
@@ -128,7 +186,7 @@ __device__ __forceinline__ void vertex_program(int kind, const DevUniforms& u, F
     // Model::normal renormalises (SR/model.cpp:108-111); ObjectToWorldNormal
     // is the row vector (n, 1) times model^-1 (SR/IShader.h:71-75)
     F3 n = normalize3(n_obj);
-    const float* mi = u.model_inv;
+    UniformRow mi = u.model_inv;
     float wn2 = 0.0f; wn2 += mi[14] * 1.0f; wn2 += mi[10] * n.z; wn2 += mi[6] * n.y; wn2 += mi[2] * n.x;
     float wn1 = 0.0f; wn1 += mi[13] * 1.0f; wn1 += mi[9] * n.z; wn1 += mi[5] * n.y; wn1 += mi[1] * n.x;
     float wn0 = 0.0f; wn0 += mi[12] * 1.0f; wn0 += mi[8] * n.z; wn0 += mi[4] * n.y; wn0 += mi[0] * n.x;
@@ -157,7 +215,7 @@ struct ShadowMap {
     const float* depth;
     int w, h;
 };
-__device__ __forceinline__ int is_in_shadow(const DevUniforms& u, const ShadowMap& sm, const float dp[4], float n_dot_l) {
+__device__ __forceinline__ int is_in_shadow(Uniforms u, const ShadowMap& sm, const float dp[4], float n_dot_l) {
     if (u.enable_shadow && sm.color) {
         float widht = (float)sm.w, height = (float)sm.h;
         float ndx = dp[0] / dp[3], ndy = dp[1] / dp[3];
@@ -171,15 +229,15 @@ __device__ __forceinline__ int is_in_shadow(const DevUniforms& u, const ShadowMa
         int px = (int)posx, py = (int)posy;
         size_t texel = (size_t)py * (size_t)sm.w + (size_t)px;
         float closest_depth;
-        if (u.shadow_fp32) closest_depth = 2.f * sm.depth[texel] - 1.f;
-        else closest_depth = (float)(sm.color[texel] & 0xFF) / 255.f;
+        if (u.shadow_fp32) closest_depth = 2.f * gld(sm.depth + texel) - 1.f;
+        else closest_depth = (float)(gld(sm.color + texel) & 0xFF) / 255.f;
         return current_depth < closest_depth;
     }
     return 1;
 }
 
 // Shared tail of BlinnShader/NormalMapShader::fragment (SR/IShader.cpp:97-108, 150-161).
-__device__ __forceinline__ C3 blinn_tail(const DevUniforms& u, const Tex& diffuse, const ShadowMap& sm, F3 wp, F3 n,
+__device__ __forceinline__ C3 blinn_tail(Uniforms u, const Tex& diffuse, const ShadowMap& sm, F3 wp, F3 n,
                                          float tu, float tv) {
     F3 L = f3(u.light_dir[0], u.light_dir[1], u.light_dir[2]);
     C3 albedo = cmulc(tex_diffuse(diffuse, tu, tv), c3p(u.mat_color));
@@ -199,7 +257,7 @@ __device__ __forceinline__ C3 blinn_tail(const DevUniforms& u, const Tex& diffus
 }
 
 // ---- fragment programs (SR/IShader.cpp:11-180) on the interpolated varyings ----
-__device__ __forceinline__ C3 fragment_program(int kind, const DevUniforms& u, const float* vy, const Tex& diffuse,
+__device__ __forceinline__ C3 fragment_program(int kind, Uniforms u, const float* vy, const Tex& diffuse,
                                                const Tex& normal, const ShadowMap& sm) {
     const C3 white = c3(1.f, 1.f, 1.f);
     switch (kind) {

@@ -194,15 +194,15 @@ __device__ __forceinline__ bool inside_plane(const float* c, int plane) {
 // k_vertex
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(BLOCK) void k_vertex(const PassDesc* passes, int npasses, Workspace ws) {
+__global__ __launch_bounds__(BLOCK) void k_vertex(const PassDesc* __restrict__ passes, int npasses, Workspace ws) {
     int gtid = blockIdx.x * BLOCK + threadIdx.x;
     int gsize = gridDim.x * BLOCK;
-    for (int i = gtid; i < ws.ntiles; i += gsize) ws.tile_count[i] = 0;
-    if (gtid < STATUS_SEQ_WORDS) ws.status[gtid] = 0;
-    if (gtid < 2) ws.qstate[gtid] = 0;
+    for (int i = gtid; i < ws.ntiles; i += gsize) gst(ws.tile_count + i, 0u);
+    if (gtid < STATUS_SEQ_WORDS) gst(ws.status + gtid, 0u);
+    if (gtid < 2) gst(ws.qstate + gtid, 0u);
 
     const PassDesc& P = passes[find_pass<0>(passes, npasses, blockIdx.x)];
-    const DevUniforms& u = *P.u;
+    Uniforms u = uniforms_of(P.u);
     int base = (blockIdx.x - P.vblock0) * BLOCK * VPT;
     bool need_n = P.kind != SHADER_SHADOW && P.kind != SHADER_TEXTURE;
     bool need_uv = P.kind != SHADER_SHADOW && P.kind != SHADER_GROUND && P.kind != SHADER_TOON;
@@ -216,11 +216,11 @@ __global__ __launch_bounds__(BLOCK) void k_vertex(const PassDesc* passes, int np
         nrm[k] = f3(0.f, 0.f, 0.f);
         tu[k] = tv[k] = 0.f;
         if (v < P.nverts) {
-            pos[k] = f3(P.px[v], P.py[v], P.pz[v]);
-            if (need_n) nrm[k] = f3(P.nx[v], P.ny[v], P.nz[v]);
+            pos[k] = f3(gld(P.px + v), gld(P.py + v), gld(P.pz + v));
+            if (need_n) nrm[k] = f3(gld(P.nx + v), gld(P.ny + v), gld(P.nz + v));
             if (need_uv) {
-                tu[k] = P.tu[v];
-                tv[k] = P.tv[v];
+                tu[k] = gld(P.tu + v);
+                tv[k] = gld(P.tv + v);
             }
         }
     }
@@ -236,22 +236,22 @@ __global__ __launch_bounds__(BLOCK) void k_vertex(const PassDesc* passes, int np
 #pragma unroll
             for (int pl = 0; pl < 7; pl++)
                 if (!inside_plane(clip, pl)) code |= 1u << pl;
-            P.code[v] = (uint8_t)code;
+            gst(P.code + v, (uint8_t)code);
             // clip_pos / clip_pos[3] and 1 / clip_pos[3] (SR/graphics.cpp:317, 336)
             float4 r;
             r.w = 1 / clip[3];
             r.z = clip[2] / clip[3];
             r.y = clip[1] / clip[3];
             r.x = clip[0] / clip[3];
-            P.vr[v] = r;
+            gst4(P.vr + v, r);
             if (P.nvary == 8) {
                 float4* d = reinterpret_cast<float4*>(P.vs + (size_t)v * 8);
-                d[0] = make_float4(vary[0], vary[1], vary[2], vary[3]);
-                d[1] = make_float4(vary[4], vary[5], vary[6], vary[7]);
+                gst4(d, make_float4(vary[0], vary[1], vary[2], vary[3]));
+                gst4(d + 1, make_float4(vary[4], vary[5], vary[6], vary[7]));
             } else if (P.nvary == 2) {
-                reinterpret_cast<float2*>(P.vs)[v] = make_float2(vary[0], vary[1]);
+                gst2(reinterpret_cast<float2*>(P.vs) + v, make_float2(vary[0], vary[1]));
             } else {
-                P.vs[v] = vary[0];
+                gst(P.vs + v, vary[0]);
             }
         }
     }
@@ -322,7 +322,7 @@ __device__ __forceinline__ uint32_t wave_reserve(uint32_t* counters, int t, bool
         bool match = pending && t == lt;
         uint64_t same = __ballot(match);
         uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&counters[lt], (uint32_t)__popcll(same));
+        if (lane == leader) base = gadd(counters + lt, (uint32_t)__popcll(same));
         base = __shfl(base, leader);
         if (match) {
             mine = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
@@ -477,7 +477,7 @@ __device__ __forceinline__ bool dies_at_first_crossed_plane(uint32_t c0, uint32_
 // ws.seg_count); faces that clipping would remove are dropped; the others are
 // queued for k_clip with one global atomic per workgroup.
 template <int BITS, int TS>
-__global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* passes, int npasses, Workspace ws) {
+__global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* __restrict__ passes, int npasses, Workspace ws) {
     constexpr int SLOTS = 1 << BITS;
     __shared__ uint64_t s_queue[BLOCK * FPT];
     __shared__ uint32_t s_keys[SLOTS], s_cnt[SLOTS];   // tile counts of the workgroup's triangles
@@ -581,7 +581,7 @@ __global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* passes, int npa
 // (after the setup segments of ws.tris) are reserved once per workgroup and
 // round.
 template <int TS>
-__global__ __launch_bounds__(CLIP_THREADS) void k_clip(const PassDesc* passes, Workspace ws) {
+__global__ __launch_bounds__(CLIP_THREADS) void k_clip(const PassDesc* __restrict__ passes, Workspace ws) {
     __shared__ float buf[CLIP_FACES][2][POLY_MAX][CLIP_NF];
     __shared__ uint32_t s_npoly, s_ntri, s_poly0, s_tri0;
     uint32_t count = min(ws.status[STATUS_CLIPQ], ws.clipq_cap);
@@ -606,7 +606,7 @@ __global__ __launch_bounds__(CLIP_THREADS) void k_clip(const PassDesc* passes, W
             F3 nn = need_n ? f3(P.nx[v], P.ny[v], P.nz[v]) : f3(0.f, 0.f, 0.f);
             float clip[4], vary[8];
             for (int c = 0; c < 8; c++) vary[c] = 0.0f;
-            vertex_program(P.kind, *P.u, p, nn, P.tu[v], P.tv[v], clip, vary);
+            vertex_program(P.kind, uniforms_of(P.u), p, nn, P.tu[v], P.tv[v], clip, vary);
             for (int c = 0; c < 4; c++) buf[g][0][li][c] = clip[c];
             for (int c = 0; c < POLY_VARY; c++) buf[g][0][li][4 + c] = vary[c];
         }
@@ -718,8 +718,8 @@ __device__ __forceinline__ void bin_record(const PassDesc* passes, const Workspa
         int wt0 = __shfl(tile0, src), wtx = __shfl(tiles_x, src);
         uint32_t we = __shfl(r.x, src);
         for (int k = __lane_id(); k < wn; k += 64) {
-            uint32_t pos = atomicAdd(&ws.tile_cursor[wt0 + (wy0 + k / wnx) * wtx + wx0 + k % wnx], 1u);
-            if (pos < ws.bin_cap) ws.bins[pos] = we;
+            uint32_t pos = gadd(ws.tile_cursor + (wt0 + (wy0 + k / wnx) * wtx + wx0 + k % wnx), 1u);
+            if (pos < ws.bin_cap) gst(ws.bins + pos, we);
         }
     }
     int ns = n > WIDE_TILES ? 0 : n;
@@ -727,7 +727,7 @@ __device__ __forceinline__ void bin_record(const PassDesc* passes, const Workspa
         bool want = k < ns;
         int gt = want ? tile0 + (ty0 + k / nx) * tiles_x + tx0 + k % nx : 0;
         uint32_t pos = wave_reserve(ws.tile_cursor, gt, want);
-        if (want && pos < ws.bin_cap) ws.bins[pos] = r.x;
+        if (want && pos < ws.bin_cap) gst(ws.bins + pos, r.x);
     }
 }
 
@@ -737,7 +737,7 @@ __device__ __forceinline__ void bin_record(const PassDesc* passes, const Workspa
 // then every pair takes a place in that range with an LDS atomic.  The other
 // workgroups stride over the clipped fan triangles listed after the segments.
 template <int BITS>
-__global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* passes, Workspace ws, int nseg) {
+__global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* __restrict__ passes, Workspace ws, int nseg) {
     constexpr int BIN_HASH = 1 << BITS;
     if (ws.status[STATUS_SEQ_FLAGS] & FLAG_NO_RASTER) return;   // nothing is rasterised
     if ((int)blockIdx.x < nseg) {
@@ -756,13 +756,13 @@ __global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* passes, Workspace
         // listed tiles are distinct, so an inserted slot is this thread's:
         // it reserves the tile's range of the bin at once
         for (uint32_t i = threadIdx.x; i < nlist; i += BLOCK) {
-            uint2 e = list[i];
+            uint2 e = gld2(list + i);
             int sl = bin_slot<BITS>(keys, e.x, true);
-            if (sl >= 0) cnt[sl] = atomicAdd(&ws.tile_cursor[e.x], e.y);
+            if (sl >= 0) cnt[sl] = gadd(ws.tile_cursor + e.x, e.y);
         }
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
-            uint4 r = rec[i];
+            uint4 r = gld4(rec + i);
             const PassDesc& P = passes[r.y];
             int tx0 = (int)(r.z & 0xFFFFu), ty0 = (int)(r.z >> 16);
             int tx1 = (int)(r.w & 0xFFFFu), ty1 = (int)(r.w >> 16);
@@ -771,21 +771,21 @@ __global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* passes, Workspace
                 for (int tx = tx0; tx <= tx1; tx++) {
                     uint32_t gt = (uint32_t)(P.tile0 + ty * P.tiles_x + tx);
                     int sl = bin_slot<BITS>(keys, gt, false);
-                    uint32_t pos = sl >= 0 ? atomicAdd(&cnt[sl], 1u) : atomicAdd(&ws.tile_cursor[gt], 1u);
-                    if (pos < ws.bin_cap) ws.bins[pos] = r.x;
+                    uint32_t pos = sl >= 0 ? atomicAdd(&cnt[sl], 1u) : gadd(ws.tile_cursor + gt, 1u);
+                    if (pos < ws.bin_cap) gst(ws.bins + pos, r.x);
                 }
         }
         // wide triangles, one per wave at a time, their tiles spread over its lanes
         const uint16_t* wide = ws.seg_wide + (size_t)blockIdx.x * (BLOCK * FPT);
         for (uint32_t w = threadIdx.x >> 6; w < nw; w += BLOCK / 64) {
-            uint4 r = rec[wide[w]];
+            uint4 r = gld4(rec + gld(wide + w));
             const PassDesc& P = passes[r.y];
             int tx0 = (int)(r.z & 0xFFFFu), ty0 = (int)(r.z >> 16);
             int nx = (int)(r.w & 0xFFFFu) - tx0 + 1;
             int nt = nx * ((int)(r.w >> 16) - ty0 + 1);
             for (int k = threadIdx.x & 63; k < nt; k += 64) {
-                uint32_t pos = atomicAdd(&ws.tile_cursor[P.tile0 + (ty0 + k / nx) * P.tiles_x + tx0 + k % nx], 1u);
-                if (pos < ws.bin_cap) ws.bins[pos] = r.x;
+                uint32_t pos = gadd(ws.tile_cursor + (P.tile0 + (ty0 + k / nx) * P.tiles_x + tx0 + k % nx), 1u);
+                if (pos < ws.bin_cap) gst(ws.bins + pos, r.x);
             }
         }
         return;
@@ -795,7 +795,7 @@ __global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* passes, Workspace
     for (uint32_t base = (blockIdx.x - (uint32_t)nseg) * BLOCK; base < count; base += stride) {
         uint32_t i = base + threadIdx.x;
         bool have = i < count;
-        bin_record(passes, ws, have, have ? ws.tris[ws.seg_region + i] : make_uint4(0, 0, 0, 0));
+        bin_record(passes, ws, have, have ? gld4(ws.tris + (ws.seg_region + i)) : make_uint4(0, 0, 0, 0));
     }
 }
 
@@ -965,14 +965,14 @@ __device__ __forceinline__ bool load_tri(const PassDesc& P, const Workspace& ws,
     float4 a, b, c;
     if (!(e & CLIPPED_BIT)) {
         int f = (int)e;
-        int i0 = P.idx[3 * f], i1 = P.idx[3 * f + 1], i2 = P.idx[3 * f + 2];
-        a = P.vr[i0]; b = P.vr[i1]; c = P.vr[i2];
+        int i0 = gld(P.idx + 3 * f), i1 = gld(P.idx + 3 * f + 1), i2 = gld(P.idx + 3 * f + 2);
+        a = gld4(P.vr + i0); b = gld4(P.vr + i1); c = gld4(P.vr + i2);
         order = (uint32_t)f << 4;
     } else {
         uint32_t pid = (e & ~CLIPPED_BIT) >> FAN_BITS, j = e & ((1u << FAN_BITS) - 1u);
         const float4* pv = ws.pool_vr + (size_t)pid * POLY_MAX;
-        a = pv[0]; b = pv[j + 1]; c = pv[j + 2];
-        order = ((uint32_t)ws.pool_face[pid] << 4) + 1u + j;
+        a = gld4(pv); b = gld4(pv + j + 1); c = gld4(pv + j + 2);
+        order = ((uint32_t)gld(ws.pool_face + pid) << 4) + 1u + j;
     }
     return tri_setup(a, b, c, P.width, P.height, t);
 }
@@ -1012,7 +1012,7 @@ __device__ void sweep(TileLds<TS, RB>& L, const PassDesc& P, const Workspace& ws
         bool ok = false;
         TriS t;
         uint32_t order = 0;
-        if (base + tid < cnt) ok = load_tri(P, ws, ws.bins[beg + base + tid], t, order);
+        if (base + tid < cnt) ok = load_tri(P, ws, gld(ws.bins + (beg + base + tid)), t, order);
         int bx0 = 0, bx1 = -1, by0 = 0, by1 = -1;
         if (ok) {
             bx0 = max(t.x0, xb) - xb; bx1 = min(t.x1, xb + EDGE - 1) - xb;
@@ -1086,15 +1086,15 @@ __device__ __forceinline__ void shade_pixel(const PassDesc& P, const Workspace& 
     const float *sa, *sb, *sc;
     int nv;
     if (slot == 0) {
-        int i0 = P.idx[3 * f], i1 = P.idx[3 * f + 1], i2 = P.idx[3 * f + 2];
-        va = P.vr[i0]; vb = P.vr[i1]; vc = P.vr[i2];
+        int i0 = gld(P.idx + 3 * f), i1 = gld(P.idx + 3 * f + 1), i2 = gld(P.idx + 3 * f + 2);
+        va = gld4(P.vr + i0); vb = gld4(P.vr + i1); vc = gld4(P.vr + i2);
         nv = P.nvary;
         sa = P.vs + (size_t)i0 * nv; sb = P.vs + (size_t)i1 * nv; sc = P.vs + (size_t)i2 * nv;
     } else {
-        int pid = P.poly_of_face[f];
+        int pid = gld(P.poly_of_face + f);
         uint32_t j = slot - 1;
         size_t base = (size_t)pid * POLY_MAX;
-        va = ws.pool_vr[base]; vb = ws.pool_vr[base + j + 1]; vc = ws.pool_vr[base + j + 2];
+        va = gld4(ws.pool_vr + base); vb = gld4(ws.pool_vr + base + j + 1); vc = gld4(ws.pool_vr + base + j + 2);
         nv = POLY_VARY;
         sa = ws.pool_vs + base * POLY_VARY;
         sb = ws.pool_vs + (base + j + 1) * POLY_VARY;
@@ -1110,7 +1110,7 @@ __device__ __forceinline__ void shade_pixel(const PassDesc& P, const Workspace& 
     float vy[8];
     for (int c = 0; c < 8; c++) {
         if (c < cnt) {
-            float sum = sa[c] * k0 + sb[c] * k1 + sc[c] * k2;
+            float sum = gld(sa + c) * k0 + gld(sb + c) * k1 + gld(sc + c) * k2;
             vy[c] = sum * normalizer;
         } else {
             vy[c] = 0.0f;
@@ -1120,18 +1120,18 @@ __device__ __forceinline__ void shade_pixel(const PassDesc& P, const Workspace& 
     ShadowMap sm{P.shadow, P.shadow_depth, P.sw, P.sh};
     size_t pix = (size_t)py * P.width + px;
     // the fp32 extension's shadow pass is depth-only
-    if (!(P.kind == SHADER_SHADOW && P.u->shadow_fp32)) {
-        C3 col = fragment_program(P.kind, *P.u, vy, dt, nt, sm);
-        uint32_t alpha = P.clear ? P.clear_rgba : P.color[pix];
-        P.color[pix] = pack_rgb(col, alpha);
+    if (!(P.kind == SHADER_SHADOW && uniforms_of(P.u).shadow_fp32)) {
+        C3 col = fragment_program(P.kind, uniforms_of(P.u), vy, dt, nt, sm);
+        uint32_t alpha = P.clear ? P.clear_rgba : gld(P.color + pix);
+        gst(P.color + pix, pack_rgb(col, alpha));
     } else if (P.clear) {
-        P.color[pix] = P.clear_rgba;
+        gst(P.color + pix, P.clear_rgba);
     }
     // A NaN depth is stored as the x86 default NaN (quiet, sign set): that is
     // what the reference's own arithmetic produces for it (an invalid
     // operation such as inf/inf yields that pattern and NaN operands pass it on).
-    P.depth[pix] = (z != z) ? __uint_as_float(0xFFC00000u) : z;
-    if (P.winner) P.winner[pix] = order;
+    gst(P.depth + pix, (z != z) ? __uint_as_float(0xFFC00000u) : z);
+    if (P.winner) gst(P.winner + pix, order);
 }
 
 // Workgroups are dispatched to the 8 XCDs round-robin by blockIdx.  The
@@ -1165,8 +1165,8 @@ __device__ void raster_tile(TileLds<TS, RB>& L, const PassDesc& P, const Workspa
     int tid = threadIdx.x;
     int W = P.width, H = P.height;
 
-    uint32_t beg = ws.tile_offset[gt];
-    uint32_t cnt = ws.tile_count[gt];
+    uint32_t beg = gld(ws.tile_offset + gt);
+    uint32_t cnt = gld(ws.tile_count + gt);
     if (beg >= ws.bin_cap) cnt = 0;
     else if (cnt > ws.bin_cap - beg) cnt = ws.bin_cap - beg;
     if (cnt == 0) {   // uniform: no fragment, the clear only
@@ -1175,10 +1175,10 @@ __device__ void raster_tile(TileLds<TS, RB>& L, const PassDesc& P, const Workspa
             if (px >= W || py >= H) continue;
             size_t pix = (size_t)py * W + px;
             if (P.clear) {
-                P.color[pix] = P.clear_rgba;
-                P.depth[pix] = P.clear_depth;
+                gst(P.color + pix, P.clear_rgba);
+                gst(P.depth + pix, P.clear_depth);
             }
-            if (P.winner) P.winner[pix] = NO_WINNER;
+            if (P.winner) gst(P.winner + pix, NO_WINNER);
         }
         return;
     }
@@ -1186,7 +1186,7 @@ __device__ void raster_tile(TileLds<TS, RB>& L, const PassDesc& P, const Workspa
     for (int i = tid; i < PIX; i += RB) {
         int px = xb + (i & (EDGE - 1)), py = yb + (i >> TS);
         uint64_t k = ~0ull;
-        if (px < W && py < H) k = init_key(P.clear ? P.clear_depth : P.depth[(size_t)py * W + px]);
+        if (px < W && py < H) k = init_key(P.clear ? P.clear_depth : gld(P.depth + ((size_t)py * W + px)));
         L.key[i] = k;
         L.nan_order[i] = 0u;
     }
@@ -1213,10 +1213,10 @@ __device__ void raster_tile(TileLds<TS, RB>& L, const PassDesc& P, const Workspa
         size_t pix = (size_t)py * W + px;
         if (lo == 0xFFFFFFFFu) {
             if (P.clear) {
-                P.color[pix] = P.clear_rgba;
-                P.depth[pix] = P.clear_depth;
+                gst(P.color + pix, P.clear_rgba);
+                gst(P.depth + pix, P.clear_depth);
             }
-            if (P.winner) P.winner[pix] = NO_WINNER;
+            if (P.winner) gst(P.winner + pix, NO_WINNER);
             continue;
         }
         shade_pixel(P, ws, 0xFFFFFFFEu - lo, px, py);
@@ -1225,11 +1225,23 @@ __device__ void raster_tile(TileLds<TS, RB>& L, const PassDesc& P, const Workspa
     if (P.stats) stat_add(P, ST_PIXELS, written);
 }
 
+// Waves per SIMD that the raster's LDS allows (160 KiB per CU, 4 SIMDs, at
+// most 8 waves each).  The kernels ask the compiler for that occupancy: the
+// persistent raster, whose tile loop keeps more values live, would otherwise
+// take 78-84 VGPRs (6 waves per SIMD) where its LDS allows 8.
+template <int TS, int RB>
+constexpr int raster_occupancy() {
+    constexpr int lds = (int)sizeof(TileLds<TS, RB>) + 16;
+    constexpr int w = (163840 / lds) * (RB / 64) / 4;
+    return w < 1 ? 1 : (w > 8 ? 8 : w);
+}
+
 // One workgroup per tile of one dependency group (passes of the group, their
 // tiles numbered from 0 by tblock0), XCD-banded.  GROUP only names the launch
 // (the shadow passes' group 0, the main passes' group 1) in profiles.
 template <int TS, int GROUP, int RB>
-__global__ __launch_bounds__(RB) void k_raster(const PassDesc* passes, int npasses, Workspace ws) {
+__global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(raster_occupancy<TS, RB>())))
+void k_raster(const PassDesc* __restrict__ passes, int npasses, Workspace ws) {
     __shared__ TileLds<TS, RB> L;
     // an incomplete sequence leaves its targets untouched: the host renders it again
     if (ws.status[STATUS_SEQ_FLAGS] & FLAG_NO_RASTER) return;
@@ -1269,7 +1281,8 @@ __device__ __forceinline__ int pass_of_item(const PassDesc* P, int n, uint32_t i
 }
 
 template <int TS, int RB>
-__global__ __launch_bounds__(RB) void k_raster_q(const PassDesc* passes, int npasses, Workspace ws) {
+__global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(raster_occupancy<TS, RB>())))
+void k_raster_q(const PassDesc* __restrict__ passes, int npasses, Workspace ws) {
     __shared__ TileLds<TS, RB> L;
     __shared__ uint32_t s_first;
     if (ws.status[STATUS_SEQ_FLAGS] & FLAG_NO_RASTER) return;
@@ -1279,7 +1292,8 @@ __global__ __launch_bounds__(RB) void k_raster_q(const PassDesc* passes, int npa
     for (;;) {
         if (tid == 0) s_first = atomicAdd(&ws.qstate[0], RQ_CHUNK);
         __syncthreads();
-        uint32_t first = s_first;
+        // uniform: the tile's pass, bounds and tables then live in SGPRs
+        uint32_t first = __builtin_amdgcn_readfirstlane(s_first);
         __syncthreads();
         uint32_t end = min(first + RQ_CHUNK, ws.q_items);
         if (!published && (first >= ws.q_g0 || first >= ws.q_items)) {
