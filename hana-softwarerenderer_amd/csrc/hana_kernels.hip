@@ -957,8 +957,8 @@ struct TileLds {
     float ax[RB], ay[RB], s0x[RB], s0y[RB], s1x[RB], s1y[RB], uz[RB];
     float z0[RB], z1[RB], z2[RB];
     uint32_t lo[RB], box[RB];
-    int big[RB];
-    int nbig, head, nan_seen;
+    int big[RB], med[RB];
+    int nbig, head, nmed, mhead, nan_seen;
 };
 
 __device__ __forceinline__ bool load_tri(const PassDesc& P, const Workspace& ws, uint32_t e, TriS& t, uint32_t& order) {
@@ -1000,8 +1000,27 @@ __device__ __forceinline__ void frag_resolve(TileLds<TS, RB>& L, const TriS& t, 
     }
 }
 
-// One sweep over the tile's bin: small triangles one per lane, larger ones
-// one per wave with the lanes spread over their candidate pixels.
+// A triangle staged in LDS by lane s of the sweep, with its candidate box.
+template <int TS, int RB>
+__device__ __forceinline__ TriS staged_tri(const TileLds<TS, RB>& L, int s, uint32_t& lo, int& x0, int& y0, int& wdt,
+                                           int& npix) {
+    TriS t;
+    t.ax = L.ax[s]; t.ay = L.ay[s]; t.s0x = L.s0x[s]; t.s0y = L.s0y[s];
+    t.s1x = L.s1x[s]; t.s1y = L.s1y[s]; t.uz = L.uz[s];
+    t.z0 = L.z0[s]; t.z1 = L.z1[s]; t.z2 = L.z2[s];
+    lo = L.lo[s];
+    uint32_t box = L.box[s];
+    x0 = box & 0xFF;
+    y0 = (box >> 16) & 0xFF;
+    wdt = (int)((box >> 8) & 0xFF) - x0 + 1;
+    npix = wdt * ((int)(box >> 24) - y0 + 1);
+    return t;
+}
+
+// One sweep over the tile's bin: small triangles one per lane (in passes that
+// share them, those of more than HANA_MED_PIX candidates outside the reject
+// pass HANA_MED_LANES lanes each), larger ones one per wave with the lanes
+// spread over their candidate pixels.
 template <int TS, int RB>
 __device__ void sweep(TileLds<TS, RB>& L, const PassDesc& P, const Workspace& ws, uint32_t beg, uint32_t cnt, int xb, int yb,
                       bool second) {
@@ -1023,7 +1042,7 @@ __device__ void sweep(TileLds<TS, RB>& L, const PassDesc& P, const Workspace& ws
         if (ok) {
             int wdt = bx1 - bx0 + 1;
             int npix = wdt * (by1 - by0 + 1);
-            if (npix <= SMALL_PIX) {
+            if (npix <= SMALL_PIX && (!P.share_small || npix <= HANA_MED_PIX || (er && npix <= HANA_ER_MAX_PIX))) {
                 if (er && npix <= HANA_ER_MAX_PIX) {
                     // cheap pass over the candidates, then the exact test on the survivors
                     uint32_t mask = 0;
@@ -1047,25 +1066,39 @@ __device__ void sweep(TileLds<TS, RB>& L, const PassDesc& P, const Workspace& ws
                 L.z0[tid] = t.z0; L.z1[tid] = t.z1; L.z2[tid] = t.z2;
                 L.lo[tid] = lo;
                 L.box[tid] = (uint32_t)bx0 | ((uint32_t)bx1 << 8) | ((uint32_t)by0 << 16) | ((uint32_t)by1 << 24);
-                int q = atomicAdd(&L.nbig, 1);
-                L.big[q] = tid;
+                if (npix <= SMALL_PIX) L.med[atomicAdd(&L.nmed, 1)] = tid;
+                else L.big[atomicAdd(&L.nbig, 1)] = tid;
             }
         }
         __syncthreads();
+        // shared small triangles: 64 / MED_LANES per wave and round
+        for (;;) {
+            constexpr int PER_WAVE = 64 / HANA_MED_LANES;
+            int q = 0;
+            if (lane == 0) q = atomicAdd(&L.mhead, PER_WAVE);
+            q = __shfl(q, 0);
+            int nmed = L.nmed;
+            if (q >= nmed) break;
+            int m = q + lane / HANA_MED_LANES;
+            if (m < nmed) {
+                uint32_t mlo;
+                int x0, y0, wdt, npix;
+                TriS mt = staged_tri<TS, RB>(L, L.med[m], mlo, x0, y0, wdt, npix);
+                uint32_t magic = tile_div_magic(wdt);
+                for (int k = lane % HANA_MED_LANES; k < npix; k += HANA_MED_LANES) {
+                    int row = tile_div(k, magic);
+                    frag_resolve<TS, RB>(L, mt, mlo, xb, yb, x0 + k - row * wdt, y0 + row, second);
+                }
+            }
+        }
         for (;;) {
             int q = 0;
             if (lane == 0) q = atomicAdd(&L.head, 1);
             q = __shfl(q, 0);
             if (q >= L.nbig) break;
-            int s = L.big[q];
-            TriS bt;
-            bt.ax = L.ax[s]; bt.ay = L.ay[s]; bt.s0x = L.s0x[s]; bt.s0y = L.s0y[s];
-            bt.s1x = L.s1x[s]; bt.s1y = L.s1y[s]; bt.uz = L.uz[s];
-            bt.z0 = L.z0[s]; bt.z1 = L.z1[s]; bt.z2 = L.z2[s];
-            uint32_t blo = L.lo[s], box = L.box[s];
-            int x0 = box & 0xFF, x1 = (box >> 8) & 0xFF, y0 = (box >> 16) & 0xFF, y1 = box >> 24;
-            int wdt = x1 - x0 + 1;
-            int npix = wdt * (y1 - y0 + 1);
+            uint32_t blo;
+            int x0, y0, wdt, npix;
+            TriS bt = staged_tri<TS, RB>(L, L.big[q], blo, x0, y0, wdt, npix);
             uint32_t magic = tile_div_magic(wdt);
             for (int k = lane; k < npix; k += 64) {
                 int row = tile_div(k, magic);
@@ -1073,7 +1106,7 @@ __device__ void sweep(TileLds<TS, RB>& L, const PassDesc& P, const Workspace& ws
             }
         }
         __syncthreads();
-        if (tid == 0) { L.nbig = 0; L.head = 0; }
+        if (tid == 0) { L.nbig = 0; L.head = 0; L.nmed = 0; L.mhead = 0; }
         __syncthreads();
     }
 }
@@ -1190,7 +1223,7 @@ __device__ void raster_tile(TileLds<TS, RB>& L, const PassDesc& P, const Workspa
         L.key[i] = k;
         L.nan_order[i] = 0u;
     }
-    if (tid == 0) { L.nbig = 0; L.head = 0; L.nan_seen = 0; }
+    if (tid == 0) { L.nbig = 0; L.head = 0; L.nmed = 0; L.mhead = 0; L.nan_seen = 0; }
     __syncthreads();
 
     sweep<TS, RB>(L, P, ws, beg, cnt, xb, yb, false);

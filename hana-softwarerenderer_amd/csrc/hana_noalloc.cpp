@@ -130,7 +130,8 @@ extern "C" int hana_render_frame_noalloc(hana_ctx* c, const hana_frame_desc* f) 
         d.width = t->w; d.height = t->h;
         d.tiles_x = (t->w + tedge - 1) / tedge;
         d.tiles_y = (t->h + tedge - 1) / tedge;
-        if ((int64_t)t->w * t->h > 4 * (int64_t)m->F) small_tris = false;
+        d.share_small = (int64_t)t->w * t->h > 4 * (int64_t)m->F;
+        if (d.share_small) small_tris = false;
         size_t vpad = (V + vchunk - 1) / vchunk * vchunk;
         voff[i] = tv; vsoff[i] = tvs; foff[i] = tf;
         tv += vpad;

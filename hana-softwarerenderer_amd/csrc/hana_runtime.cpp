@@ -186,7 +186,8 @@ int build_plan(hana_ctx* c, std::vector<PassSpec> specs, Plan& pl, const Tuning&
         d.width = t->w; d.height = t->h;
         d.tiles_x = (t->w + tedge - 1) / tedge;
         d.tiles_y = (t->h + tedge - 1) / tedge;
-        if ((int64_t)t->w * t->h > 4 * (int64_t)m->F) pl.small_tris = false;
+        d.share_small = (int64_t)t->w * t->h > 4 * (int64_t)m->F;
+        if (d.share_small) pl.small_tris = false;
         const size_t vchunk = (size_t)BLOCK * VPT, fchunk = (size_t)BLOCK * FPT;
         size_t vpad = (V + vchunk - 1) / vchunk * vchunk;
         // slices: pointers are resolved by bind_plan once the arena exists
@@ -1401,7 +1402,11 @@ int hana_batch_get_tuning(hana_ctx* c, const hana_batch* b, hana_tuning* out) {
 // the tile size, then the raster form (per-tile workgroups or a persistent
 // grid of 2, 4 or 8 workgroups per CU), then the waves per raster workgroup,
 // then the number of parts; each keeps the best value found so far for the
-// others.
+// others.  A candidate replaces the best only if it is faster by more than
+// kAutotuneMargin, so that run-to-run noise between near-equal tunings does
+// not change the choice.
+static constexpr double kAutotuneMargin = 0.02;
+
 int hana_batch_autotune(hana_ctx* c, hana_batch* b, int use_graph, int reps, hana_tuning* chosen) {
     if (!c || !b || reps < 1) return HANA_E_INVALID;
     int r;
@@ -1432,7 +1437,7 @@ int hana_batch_autotune(hana_ctx* c, hana_batch* b, int use_graph, int reps, han
             field = vals[i];
             double ms = 0.0;
             if ((r = measure(t, &ms))) return r;
-            if (ms < best_ms) {
+            if (ms < best_ms * (1.0 - kAutotuneMargin)) {
                 best = t;
                 best_ms = ms;
             }

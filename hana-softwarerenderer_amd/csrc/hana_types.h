@@ -35,6 +35,18 @@ constexpr int BLOCK = 256;                 // threads per workgroup (4 wave64)
 #ifndef HANA_ER_MIN_ENTRIES
 #define HANA_ER_MIN_ENTRIES 320
 #endif
+// In passes of over 4 target pixels per face, small triangles of more than
+// HANA_MED_PIX candidates (outside the reject pass) are shared by
+// HANA_MED_LANES lanes each instead of looped over by one: a wave's trip count
+// is then bounded by HANA_SMALL_PIX / HANA_MED_LANES.  Passes of smaller faces
+// (micro-triangle meshes) keep one lane per triangle, which their few
+// candidates suit better.
+#ifndef HANA_MED_PIX
+#define HANA_MED_PIX 4
+#endif
+#ifndef HANA_MED_LANES
+#define HANA_MED_LANES 4
+#endif
 // A raster workgroup (one tile at a time) has 2 or 4 waves: 128 or 256 lanes,
 // at most one lane per pixel of a 16x16 tile.
 static_assert(HANA_SMALL_PIX >= 1 && HANA_SMALL_PIX <= 32, "small-triangle candidates fit a 32-bit mask");
@@ -99,7 +111,8 @@ struct PassDesc {
     uint32_t* color; float* depth; uint32_t* winner;
     int32_t width, height, tiles_x, tiles_y;
     int32_t clear;               // 1: the target is cleared to clear_rgba/clear_depth first
-    uint32_t clear_rgba; float clear_depth; int32_t pad0_;
+    uint32_t clear_rgba; float clear_depth;
+    int32_t share_small;         // 1: the pass averages over 4 target pixels per face (see HANA_MED_PIX)
     // workspace slices owned by this pass
     float4* vr;                  // per vertex: ndc.x, ndc.y, ndc.z, 1/w
     float* vs;                   // per vertex: nvary varyings

@@ -243,7 +243,8 @@ int hana_batch_finalize(hana_ctx* ctx, hana_batch* b, int use_graph);
  * tuning, tile size, then raster form (per tile, or 2, 4 or 8 persistent
  * workgroups per CU), then raster waves (4, 2), then parts (1, 2, 4), one
  * dimension at a time, each
- * candidate sized, captured and timed over `reps` runs.  The frames are
+ * candidate sized, captured and timed over `reps` runs; a candidate must be
+ * more than 2 % faster than the best so far to replace it.  The frames are
  * rendered by every run.  `chosen` (nullable) receives the choice and its
  * time. */
 int hana_batch_autotune(hana_ctx* ctx, hana_batch* b, int use_graph, int reps, hana_tuning* chosen);
