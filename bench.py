@@ -43,9 +43,12 @@ sys.path[:0] = [os.path.join(REPO, "hana-softwarerenderer_amd"), os.path.join(RE
 
 METRIC = "shaded Mfrag/s + frames/s (batched vs per-frame, same GPU); p50/p99 request ms"
 HBM_PEAK_GBPS = 8000.0
-# VALU issue peak of MI355X: 256 CUs x 4 SIMDs at 2.4 GHz, one wave64 FP32
-# instruction per SIMD every 4 cycles (16 lanes a cycle; packed FP32 is off).
-VALU_PEAK_PER_S = 256 * 4 * 2.4e9 / 4
+# VALU issue peak of MI355X from its published FP32 vector peak, 157.3
+# TFLOP/s: 256 CUs x 4 SIMDs at 2.4 GHz, one wave64 instruction per SIMD
+# every 2 cycles (a wave64 FMA is 128 flops).  A 4-cycle issue model is
+# contradicted by the counters: the micro-triangle frame's shadow raster
+# retires 1.2x the wave64 VALU instructions that model allows in its time.
+VALU_PEAK_PER_S = 256 * 4 * 2.4e9 / 2
 STAGES = ("vertex", "setup", "clip", "scan", "bin", "unused", "raster_g0", "raster_g1")
 
 
@@ -647,7 +650,7 @@ def stage_of_kernel(name):
     base = re.sub(r"^void ", "", name).split("(")[0].replace("hana::", "").strip()
     if base.startswith("k_raster_q"):
         return "raster_g0"
-    m = re.match(r"k_raster<\d+, (\d)>", base)
+    m = re.match(r"k_raster<\d+, (\d)(, \d+)?>", base)
     if m:
         return "raster_g" + m.group(1)
     for k, st in (("k_vertex", "vertex"), ("k_setup", "setup"), ("k_clip", "clip"), ("k_scan", "scan"),
