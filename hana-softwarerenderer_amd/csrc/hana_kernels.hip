@@ -309,28 +309,35 @@ __device__ __forceinline__ float4 ndc_of(const float* v) {
 // ---------------------------------------------------------------------------
 
 // Reserves one slot in counters[t] for every lane with `want`, with one atomic
-// per distinct t in the wave; returns the lane's slot.  Every active lane of
-// the wave must call it together (the loop is wave-uniform).
+// per distinct t in the wave; returns the lane's slot.  The lanes are grouped
+// by t first, without any memory access; then every group's first lane issues
+// the group's atomic, and the returned bases are awaited once for all groups
+// (awaiting each group's atomic before forming the next group serialises their
+// round trips where a wave's tiles are scattered).  Every active lane of the
+// wave must call it together (the loop is wave-uniform).
 __device__ __forceinline__ uint32_t wave_reserve(uint32_t* counters, int t, bool want) {
-    uint32_t mine = 0;
+    int lane = __lane_id();
     bool pending = want;
     uint64_t mask = __ballot(pending);
-    int lane = __lane_id();
+    int leader_of = lane;
+    uint32_t rank = 0, group = 0;
     while (mask) {
         int leader = __ffsll((unsigned long long)mask) - 1;
         int lt = __shfl(t, leader);
         bool match = pending && t == lt;
         uint64_t same = __ballot(match);
-        uint32_t base = 0;
-        if (lane == leader) base = gadd(counters + lt, (uint32_t)__popcll(same));
-        base = __shfl(base, leader);
         if (match) {
-            mine = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+            leader_of = leader;
+            rank = (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
             pending = false;
         }
-        mask = __ballot(pending);
+        if (lane == leader) group = (uint32_t)__popcll(same);
+        mask &= ~same;
     }
-    return mine;
+    uint32_t base = 0;
+    if (want && leader_of == lane) base = gadd(counters + t, group);
+    base = __shfl(base, leader_of);
+    return base + rank;
 }
 
 // Adds one to counters[t] for every lane with `want`: one atomic per distinct
