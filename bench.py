@@ -43,12 +43,15 @@ sys.path[:0] = [os.path.join(REPO, "hana-softwarerenderer_amd"), os.path.join(RE
 
 METRIC = "shaded Mfrag/s + frames/s (batched vs per-frame, same GPU); p50/p99 request ms"
 HBM_PEAK_GBPS = 8000.0
-# VALU issue peak of MI355X from its published FP32 vector peak, 157.3
-# TFLOP/s: 256 CUs x 4 SIMDs at 2.4 GHz, one wave64 instruction per SIMD
-# every 2 cycles (a wave64 FMA is 128 flops).  A 4-cycle issue model is
-# contradicted by the counters: the micro-triangle frame's shadow raster
-# retires 1.2x the wave64 VALU instructions that model allows in its time.
-VALU_PEAK_PER_S = 256 * 4 * 2.4e9 / 2
+# VALU issue peak of MI355X for unpacked FP32 code (packed FP32 is off): 256
+# CUs x 4 SIMDs at 2.4 GHz, one wave64 instruction per SIMD every 4 cycles
+# (16 lanes a cycle; the published 157.3 TFLOP/s counts packed FP32, two
+# operations per lane).  SQ_INSTS_VALU counts an instruction whatever its
+# active lanes, and the micro-triangle frame's shadow raster (43 % of lanes
+# active) retires 1.2x this rate: instructions with lanes masked off must,
+# on average, issue in fewer cycles.  The fraction indicates VALU pressure;
+# it is not a bound.
+VALU_PEAK_PER_S = 256 * 4 * 2.4e9 / 4
 STAGES = ("vertex", "setup", "clip", "scan", "bin", "unused", "raster_g0", "raster_g1")
 
 

@@ -126,6 +126,64 @@ __device__ __forceinline__ bool tri_cover(const TriS& t, int px, int py, float& 
     return true;
 }
 
+// n / u.z as the hardware's IEEE division sequence computes it when its
+// scaling steps are no-ops, from the reciprocal r = cover_recip(u.z) that
+// sequence refines (the same operations in the same order, so the same bits):
+// q = n r; e = fma(-u.z, q, n); q = fma(e, r, q); e = fma(-u.z, q, n);
+// q = fma(e, r, q).  See tri_cover_fast for when that holds.
+__device__ __forceinline__ float cover_recip(float uz) {
+    float r = __builtin_amdgcn_rcpf(uz);
+    float e = fmaf(-uz, r, 1.0f);
+    return fmaf(e, r, r);
+}
+__device__ __forceinline__ float cover_quot(float n, float uz, float r) {
+    float q = n * r;
+    float e = fmaf(-uz, q, n);
+    q = fmaf(e, r, q);
+    e = fmaf(-uz, q, n);
+    return fmaf(e, r, q);
+}
+
+// tri_cover with the three divisions from one refined reciprocal.  The
+// division sequence (div_scale, rcp, four fma, div_fmas, div_fixup) scales its
+// operands only when the quotient, the reciprocal or the numerator is near
+// the end of the exponent range, and its fixup changes the result only for
+// zero, infinite or NaN operands or a quotient out of range.  With the
+// triangle's screen coordinates and edges within 2^22 and pixels within 2^20,
+// |n| <= 2^47 and 2^-7 < |u.z| <= 2^45, so for every numerator with |n| >=
+// 2^-80 neither happens and cover_quot gives the division's bits.  A zero
+// numerator gives a zero of either sign, which changes neither the coverage
+// test nor the depth (summed from +0).  Other candidates divide.
+__device__ __forceinline__ bool tri_cover_fast(const TriS& t, int px, int py, float& wx, float& wy, float& wz, float& z) {
+    float Px = (float)px, Py = (float)py;
+    float s0z = t.ax - Px;
+    float s1z = t.ay - Py;
+    float ux = t.s0y * s1z - s0z * t.s1y;
+    float uy = s0z * t.s1x - t.s0x * s1z;
+    float sum = ux + uy;
+    float ext = fmaxf(fmaxf(fmaxf(fabsf(t.ax), fabsf(t.ay)), fmaxf(fabsf(t.s0x), fabsf(t.s0y))),
+                      fmaxf(fabsf(t.s1x), fabsf(t.s1y)));
+    float r = cover_recip(t.uz);
+    // zero, or at least 2^-80 in magnitude: (|bits| << 1) - 1 >= (bits(2^-80) << 1) - 1
+    uint32_t m = min(min((__float_as_uint(sum) << 1) - 1u, (__float_as_uint(uy) << 1) - 1u),
+                     (__float_as_uint(ux) << 1) - 1u);
+    if (ext <= 0x1p22f && m >= ((uint32_t)(127 - 80) << 24) - 1u) {
+        wx = 1.f - cover_quot(sum, t.uz, r);
+        wy = cover_quot(uy, t.uz, r);
+        wz = cover_quot(ux, t.uz, r);
+    } else {
+        wx = 1.f - sum / t.uz;
+        wy = uy / t.uz;
+        wz = ux / t.uz;
+    }
+    if (wx < 0 || wy < 0 || wz < 0) return false;
+    z = 0.0f;
+    z += t.z2 * wz;
+    z += t.z1 * wy;
+    z += t.z0 * wx;
+    return true;
+}
+
 // A division-free test that rejects only pixels tri_cover rejects too (it
 // never rejects a covered one), so that the three divisions run only where a
 // fragment can result.  With |u.z| <= 2^60:
@@ -989,7 +1047,7 @@ template <int TS, int RB>
 __device__ __forceinline__ void frag_resolve(TileLds<TS, RB>& L, const TriS& t, uint32_t lo, int xb, int yb, int lx, int ly,
                                              bool second) {
     float wx, wy, wz, z;
-    if (!tri_cover(t, xb + lx, yb + ly, wx, wy, wz, z)) return;
+    if (!tri_cover_fast(t, xb + lx, yb + ly, wx, wy, wz, z)) return;
     int i = (ly << TS) | lx;
     if (!second) {
         if (z != z) {
