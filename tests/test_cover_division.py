@@ -1,0 +1,105 @@
+"""The raster's coverage quotients n / u.z from one refined reciprocal.
+
+k_raster computes the three barycentric quotients of a candidate pixel as
+q = n r; e = fma(-u.z, q, n); q = fma(e, r, q); e = fma(-u.z, q, n);
+q = fma(e, r, q), with r the reciprocal of u.z from the hardware's rcp
+(within 1 ulp) refined once, r = fma(fma(-u.z, r0, 1), r0, r0).  Those are
+the operations of the hardware's IEEE division sequence when its scaling
+steps do nothing, which the kernel ensures by taking the sequence only for
+2^-7 < |u.z| <= 2^45 and |n| >= 2^-80 (or n == 0).  On the GPU the images are
+bit-identical either way; this pins the arithmetic itself on the host: for
+every reciprocal within 1 ulp (the correctly rounded one and its two
+neighbours) the sequence returns the correctly rounded quotient, over random
+and structured operands of that range.  Host C (gcc), fmaf from libm, no GPU."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
+
+SRC = r"""
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+static uint64_t st = 0x9E3779B97F4A7C15ull;
+static uint64_t nxt(void) {   /* splitmix64 */
+    uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static float bits(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+static uint32_t ubits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+/* a float of random mantissa and sign with exponent in [lo, hi] */
+static float rnd(int lo, int hi) {
+    uint64_t r = nxt();
+    int e = lo + (int)(r % (uint64_t)(hi - lo + 1));
+    uint32_t m = (uint32_t)(r >> 20) & 0x7FFFFFu;
+    uint32_t s = (uint32_t)(r >> 63) << 31;
+    return bits(s | ((uint32_t)(e + 127) << 23) | m);
+}
+static volatile float sink;
+static float quot(float n, float uz, float r0) {
+    float e0 = fmaf(-uz, r0, 1.0f);
+    float r = fmaf(e0, r0, r0);
+    float q = n * r;
+    float e = fmaf(-uz, q, n);
+    q = fmaf(e, r, q);
+    e = fmaf(-uz, q, n);
+    return fmaf(e, r, q);
+}
+static long bad = 0, checked = 0;
+static void check(float n, float uz) {
+    float exact = n / uz;
+    float r = 1.0f / uz;
+    float rs[3] = {r, nextafterf(r, INFINITY), nextafterf(r, -INFINITY)};
+    for (int k = 0; k < 3; k++) {
+        float q = quot(n, uz, rs[k]);
+        checked++;
+        /* zero quotients may differ in sign only */
+        if (n == 0.0f ? q != 0.0f : ubits(q) != ubits(exact)) {
+            if (bad < 5) printf("MISMATCH n=%a uz=%a r=%a q=%a exact=%a\n", n, uz, rs[k], q, exact);
+            bad++;
+        }
+    }
+}
+int main(int argc, char** argv) {
+    long n_random = argc > 1 ? atol(argv[1]) : 2000000;
+    for (long i = 0; i < n_random; i++) {
+        float uz = rnd(-6, 44);                 /* 2^-6 <= |u.z| < 2^45 (0.01 < 2^-6 would pass too) */
+        if (fabsf(uz) <= 0.01f) continue;
+        check(rnd(-80, 46), uz);                /* 2^-80 <= |n| < 2^47 */
+    }
+    /* structured: quotients at and next to exact values, near 1, mantissa extremes */
+    for (long i = 0; i < n_random / 4; i++) {
+        float uz = rnd(-6, 40);
+        float k = (float)(int32_t)(nxt() % 2000001) - 1000000.0f;
+        float n = uz * k;                       /* exact multiples where representable */
+        check(n, uz);
+        check(nextafterf(n, INFINITY), uz);
+        check(nextafterf(n, -INFINITY), uz);
+        check(uz, uz);
+        check(nextafterf(uz, 0.0f), uz);
+        uint32_t ones = (ubits(uz) & 0xFF800000u) | 0x7FFFFFu;   /* all-ones mantissa divisor */
+        check(rnd(-80, 46), bits(ones));
+        check(0.0f, uz);
+        check(-0.0f, uz);
+    }
+    printf("checked %ld bad %ld\n", checked, bad);
+    return bad != 0;
+}
+"""
+
+
+def test_refined_reciprocal_quotients_are_correctly_rounded(tmp_path):
+    c = tmp_path / "cover_div.c"
+    c.write_text(SRC.replace("#include <string.h>", "#include <string.h>\n#include <stdlib.h>"))
+    exe = tmp_path / "cover_div"
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-o", str(exe), str(c), "-lm"], check=True)
+    out = subprocess.run([str(exe), "3000000"], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout[-2000:]
+    assert "bad 0" in out.stdout
