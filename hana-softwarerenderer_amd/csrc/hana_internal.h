@@ -63,7 +63,8 @@ struct Tuning {
 // Grow-only device workspace for launch sequences.
 struct Arena {
     // tables: the pass descriptors, then (at Plan::uni_off) their uniforms,
-    // uploaded by one copy per call
+    // then (at Plan::tp_off) the raster's pass lookup, uploaded by one copy
+    // per call
     DevBuf tables, vr, vs, code, pof, tile_count, tile_offset, tile_cursor, bins, tris, seg_count, scan_part;
     DevBuf seg_tiles, seg_wide, seg_info, qstate;
     DevBuf pool_vr, pool_vs, pool_face, pool_pass, pool_n, clip_queue, status, stats;
@@ -81,6 +82,11 @@ struct Plan {
     int group_first[2] = {0, 0}, group_count[2] = {0, 0}, group_blocks[2] = {0, 0};
     size_t total_v = 0, total_vs = 0, total_f = 0;
     size_t uni_off = 0;   // byte offset of the uniforms in the tables buffer
+    // k_raster's pass lookup (Workspace::tile_pass) of both groups, at byte
+    // tp_off of the tables buffer; group g's entries start at tp_first[g]
+    std::vector<uint2> tile_pass;
+    int tp_first[2] = {0, 0};
+    size_t tp_off = 0;
     int total_tiles = 0;
     bool small_tris = true;   // every pass has at most 4 target pixels per face
     int tile_shift = TILE_SHIFT_DEFAULT;

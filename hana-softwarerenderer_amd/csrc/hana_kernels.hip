@@ -1255,16 +1255,14 @@ __device__ __forceinline__ int xcd_logical_block(int b, int nb) {
 // the pass's clear.  The workgroup's threads all call it; it leaves L in use
 // until they have all returned.
 template <int TS, int RB>
-__device__ void raster_tile(TileLds<TS, RB>& L, const PassDesc& P, const Workspace& ws, int lt) {
+__device__ void raster_tile(TileLds<TS, RB>& L, const PassDesc& P, const Workspace& ws, int lt, uint32_t beg,
+                            uint32_t cnt) {
     constexpr int EDGE = 1 << TS, PIX = 1 << (2 * TS);
     int tx = lt % P.tiles_x, ty = lt / P.tiles_x;
-    int gt = P.tile0 + lt;
     int xb = tx << TS, yb = ty << TS;
     int tid = threadIdx.x;
     int W = P.width, H = P.height;
 
-    uint32_t beg = gld(ws.tile_offset + gt);
-    uint32_t cnt = gld(ws.tile_count + gt);
     if (beg >= ws.bin_cap) cnt = 0;
     else if (cnt > ws.bin_cap - beg) cnt = ws.bin_cap - beg;
     if (cnt == 0) {   // uniform: no fragment, the clear only
@@ -1323,6 +1321,13 @@ __device__ void raster_tile(TileLds<TS, RB>& L, const PassDesc& P, const Workspa
     if (P.stats) stat_add(P, ST_PIXELS, written);
 }
 
+// The bin range of global tile gt (uniform; written by earlier launches of
+// the sequence only, so read through the scalar cache).
+__device__ __forceinline__ uint2 tile_range(const Workspace& ws, int gt) {
+    return make_uint2(*(const HANA_CONST_AS uint32_t*)(ws.tile_offset + gt),
+                      *(const HANA_CONST_AS uint32_t*)(ws.tile_count + gt));
+}
+
 // Waves per SIMD that the raster's LDS allows (160 KiB per CU, 4 SIMDs, at
 // most 8 waves each).  The kernels ask the compiler for that occupancy: the
 // persistent raster, whose tile loop keeps more values live, would otherwise
@@ -1334,6 +1339,28 @@ constexpr int raster_occupancy() {
     return w < 1 ? 1 : (w > 8 ? 8 : w);
 }
 
+// The pass of raster workgroup blk of group GROUP and its global tile: one
+// scalar load of the group's lookup (Workspace::tile_pass), then, in a run of
+// workgroups that spans passes, a step per pass boundary; a binary search
+// without a lookup.
+template <int GROUP>
+__device__ __forceinline__ int raster_pass(const PassDesc* P, int n, const Workspace& ws, int blk, int& gt) {
+    const uint2* tp = ws.tile_pass[GROUP];
+    if (!tp) {
+        int p = find_pass<2>(P, n, blk);
+        gt = P[p].tile0 + blk - P[p].tblock0;
+        return p;
+    }
+    u2_vec e = *(const HANA_CONST_AS u2_vec*)(tp + (blk >> TILE_PASS_SHIFT));
+    int p = (int)(e.x & ~TILE_PASS_PURE);
+    gt = blk + (int)e.y;
+    if (!(e.x & TILE_PASS_PURE)) {
+        while (p + 1 < n && P[p + 1].tblock0 <= blk) p++;
+        gt = P[p].tile0 + blk - P[p].tblock0;
+    }
+    return p;
+}
+
 // One workgroup per tile of one dependency group (passes of the group, their
 // tiles numbered from 0 by tblock0), XCD-banded.  GROUP only names the launch
 // (the shadow passes' group 0, the main passes' group 1) in profiles.
@@ -1341,11 +1368,18 @@ template <int TS, int GROUP, int RB>
 __global__ __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(raster_occupancy<TS, RB>())))
 void k_raster(const PassDesc* __restrict__ passes, int npasses, Workspace ws) {
     __shared__ TileLds<TS, RB> L;
-    // an incomplete sequence leaves its targets untouched: the host renders it again
-    if (ws.status[STATUS_SEQ_FLAGS] & FLAG_NO_RASTER) return;
+    // the loads that decide the workgroup's work are issued before any of
+    // them is awaited: the sequence flags, the pass lookup, the bin range
+    uint32_t flags = *(const HANA_CONST_AS uint32_t*)(ws.status + STATUS_SEQ_FLAGS);
     int blk = xcd_logical_block(blockIdx.x, gridDim.x);
-    const PassDesc& P = passes[find_pass<2>(passes, npasses, blk)];
-    raster_tile<TS, RB>(L, P, ws, blk - P.tblock0);
+    int gt;
+    const PassDesc& P = passes[raster_pass<GROUP>(passes, npasses, ws, blk, gt)];
+    uint2 range = tile_range(ws, gt);
+    // an incomplete sequence leaves its targets untouched: the host renders it
+    // again.  (Bin offsets never reach 2^32 - 1; testing the range as well
+    // keeps its loads ahead of the flags' wait.)
+    if ((flags & FLAG_NO_RASTER) | (uint32_t)((range.x & range.y) == 0xFFFFFFFFu)) return;
+    raster_tile<TS, RB>(L, P, ws, blk - P.tblock0, range.x, range.y);
 }
 
 // The persistent raster: a fixed grid of workgroups takes the tiles of every
@@ -1422,7 +1456,8 @@ void k_raster_q(const PassDesc* __restrict__ passes, int npasses, Workspace ws) 
                 ready = true;
             }
             const PassDesc& P = passes[pass_of_item(passes, npasses, it)];
-            raster_tile<TS, RB>(L, P, ws, (int)(it - (uint32_t)P.tile0));
+            uint2 range = tile_range(ws, (int)it);
+            raster_tile<TS, RB>(L, P, ws, (int)(it - (uint32_t)P.tile0), range.x, range.y);
             __syncthreads();   // the tile's writes are issued and L is free again
             if (it < ws.q_g0) mine++;
         }

@@ -229,6 +229,23 @@ int build_plan(hana_ctx* c, std::vector<PassSpec> specs, Plan& pl, const Tuning&
         pl.group_count[g] = count;
         pl.group_blocks[g] = tb[g];
     }
+    // k_raster's pass of each run of 2^TILE_PASS_SHIFT workgroups (a group's
+    // passes are contiguous, in tblock0 order): one scalar load per workgroup
+    // instead of a binary search over the group's descriptors
+    pl.tile_pass.clear();
+    for (int g = 0; g < 2; g++) {
+        pl.tp_first[g] = (int)pl.tile_pass.size();
+        const int first = pl.group_first[g], cnt = pl.group_count[g], nb = pl.group_blocks[g];
+        const int run = 1 << TILE_PASS_SHIFT;
+        int p = 0;
+        for (int b0 = 0; cnt > 0 && b0 < nb; b0 += run) {
+            while (p + 1 < cnt && pl.descs[first + p + 1].tblock0 <= b0) p++;
+            const int last = std::min(b0 + run, nb) - 1;
+            const bool pure = !(p + 1 < cnt && pl.descs[first + p + 1].tblock0 <= last);
+            const PassDesc& d = pl.descs[first + p];
+            pl.tile_pass.push_back(make_uint2((uint32_t)p | (pure ? TILE_PASS_PURE : 0u), (uint32_t)(d.tile0 - d.tblock0)));
+        }
+    }
     return HANA_OK;
 }
 
@@ -238,7 +255,8 @@ int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_p
     int n = (int)pl.descs.size();
     int r;
     pl.uni_off = (sizeof(PassDesc) * std::max(n, 1) + 255) & ~(size_t)255;
-    if ((r = ensure(c, A.tables, pl.uni_off + sizeof(DevUniforms) * std::max(n, 1)))) return r;
+    pl.tp_off = (pl.uni_off + sizeof(DevUniforms) * std::max(n, 1) + 255) & ~(size_t)255;
+    if ((r = ensure(c, A.tables, pl.tp_off + sizeof(uint2) * std::max<size_t>(pl.tile_pass.size(), 1)))) return r;
     if ((r = ensure(c, A.vr, sizeof(float4) * std::max<size_t>(pl.total_v, 1)))) return r;
     if ((r = ensure(c, A.vs, sizeof(float) * std::max<size_t>(pl.total_vs, 1)))) return r;
     if ((r = ensure(c, A.code, std::max<size_t>(pl.total_v, 4)))) return r;
@@ -336,6 +354,10 @@ int bind_plan(hana_ctx* c, Plan& pl, Arena& A, uint32_t min_bins, uint32_t min_p
     ws.q_dep_from = pl.q_dep_from;
     ws.tile_shift = pl.tile_shift;
     ws.raster_waves = pl.raster_waves;
+    for (int g = 0; g < 2; g++)
+        ws.tile_pass[g] = pl.group_count[g] > 0
+                              ? reinterpret_cast<const uint2*>(A.tables.as<char>() + pl.tp_off) + pl.tp_first[g]
+                              : nullptr;
     return HANA_OK;
 }
 
@@ -445,7 +467,8 @@ int enqueue_sequence(hana_ctx* c, const Plan& pl, const Arena& A, hipStream_t s,
 // them into the arena's tables buffer (one copy).
 int upload_tables(hana_ctx* c, const Plan& pl, Arena& A, hipStream_t s) {
     size_t nd = sizeof(PassDesc) * pl.descs.size(), nu = sizeof(DevUniforms) * pl.unis.size();
-    size_t need = pl.uni_off + nu;
+    size_t nt = sizeof(uint2) * pl.tile_pass.size();
+    size_t need = pl.tp_off + nt;
     if (need > c->ring_bytes) {
         int r = sync_all(c);
         if (r) return r;
@@ -464,6 +487,7 @@ int upload_tables(hana_ctx* c, const Plan& pl, Arena& A, hipStream_t s) {
     char* h = static_cast<char*>(c->ring_host[slot]);
     std::memcpy(h, pl.descs.data(), nd);
     std::memcpy(h + pl.uni_off, pl.unis.data(), nu);
+    if (nt) std::memcpy(h + pl.tp_off, pl.tile_pass.data(), nt);
     HIP_TRY(c, hipMemcpyAsync(A.tables.p, h, need, hipMemcpyHostToDevice, s));
     HIP_TRY(c, hipEventRecord(c->ring_event[slot], s));
     c->ring_used[slot] = true;
@@ -1238,6 +1262,9 @@ static int batch_size_part(hana_ctx* c, hana_batch* b, hana_batch_part& part) {
                              hipMemcpyHostToDevice));
         HIP_TRY(c, hipMemcpy(A.tables.as<char>() + part.plan.uni_off, part.plan.unis.data(),
                              sizeof(DevUniforms) * part.plan.unis.size(), hipMemcpyHostToDevice));
+        if (!part.plan.tile_pass.empty())
+            HIP_TRY(c, hipMemcpy(A.tables.as<char>() + part.plan.tp_off, part.plan.tile_pass.data(),
+                                 sizeof(uint2) * part.plan.tile_pass.size(), hipMemcpyHostToDevice));
         // trial run: sizes the bins and the clip pool exactly
         if ((r = enqueue_sequence(c, part.plan, A, c->stream))) return r;
         uint32_t st[STATUS_WORDS];

@@ -161,7 +161,15 @@ struct Workspace {
     uint32_t q_dep_from;         // first item of a pass that samples a map drawn by group 0
     int32_t tile_shift;          // tiles are 2^tile_shift pixels a side
     int32_t raster_waves;        // waves per raster workgroup: 2 or 4
+    // per raster group, or null: for every run of 2^TILE_PASS_SHIFT
+    // workgroups of k_raster, the pass (from the group's first) of the run's
+    // first tile, with TILE_PASS_PURE set when the whole run lies in it, and
+    // that pass's tile0 - tblock0 (workgroup b of it renders global tile
+    // b + that)
+    const uint2* tile_pass[2];
 };
+constexpr int TILE_PASS_SHIFT = 6;
+constexpr uint32_t TILE_PASS_PURE = 0x80000000u;
 
 // the tile scan: SCAN_CHUNK tile counters per workgroup of SCAN_THREADS
 constexpr int SCAN_THREADS = 256;
