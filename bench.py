@@ -774,7 +774,8 @@ def main():
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--serve-cpu-sample", type=int, default=40)
     ap.add_argument("--serve-cpu-replay", type=int, default=2000)
-    ap.add_argument("--c5-all-core", action="store_true", help="also run the micro-triangle frame on every core")
+    ap.add_argument("--c5-one-core", action="store_true",
+                    help="time the micro-triangle frame's CPU leg on one core only (not on every core as well)")
     ap.add_argument("--no-fp32", action="store_true", help="skip the fp32 shadow-map variants of c2 and c5")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy (scales rocprof FETCH/WRITE_SIZE)")
@@ -858,7 +859,7 @@ def main():
             del replay
         if "c5" in wl:
             s5 = cpu_single([wl["c5"]], kind, keep=True, label="c5")
-            a5 = cpu_all("c5", [wl["c5"]], kind, repeat=W, label="c5") if args.c5_all_core else None
+            a5 = cpu_all("c5", [wl["c5"]], kind, repeat=W, label="c5") if not args.c5_one_core else None
             cpu["c5"] = {"fragments_per_frame": s5["fragments"], "images": s5["images"][:1],
                          "baseline": cpu_baseline_block(s5, a5 or dict(s5, workers=1), kind, None,
                                                         "one whole frame on one core",
