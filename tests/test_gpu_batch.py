@@ -77,6 +77,30 @@ def test_config3_full_batch(ctx):
     batch.close()
 
 
+@pytest.mark.parametrize("tile", [16, 32])
+def test_many_tiny_frames(ctx, tile):
+    """Forty frames of 8^2-40^2 pixels (1-9 tiles each) in one batch part: the
+    raster's runs of 64 workgroups each span many passes, so every workgroup
+    steps to its pass from the run's first.  Against the oracle, twice."""
+    from hana_amd.gpu import Batch, SceneOnDevice
+    tiny = sg.config3_scenes(count=40, tri_lo=50, tri_hi=2_000, px_lo=8 * 8, px_hi=40 * 40)
+    o = CPURenderer("oracle")
+    refs = [o.render_frame(sc)[0] for sc in tiny]
+    devs = [SceneOnDevice(ctx, sc) for sc in tiny]
+    batch = Batch(ctx)
+    for d in devs:
+        batch.add(d.desc())
+    batch.set_tuning(tile, 0, 1, 4)
+    batch.finalize(use_graph=True)
+    for run in range(2):
+        poison(ctx, devs)
+        batch.run()
+        for sc, (col, dep), ref in zip(tiny, read_all(ctx, devs), refs):
+            assert_depth_exact(dep, ref.depth, f"{sc.name} run {run}")
+            assert_color_ok(col, ref.color, f"{sc.name} run {run}")
+    batch.close()
+
+
 def test_batch_matches_oracle_and_per_frame(ctx, scenes, oracle_frames):
     from hana_amd.gpu import Batch, SceneOnDevice
     devs = [SceneOnDevice(ctx, s) for s in scenes]
