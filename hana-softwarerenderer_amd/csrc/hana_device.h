@@ -132,6 +132,18 @@ __device__ __forceinline__ C3 cmulc(C3 a, C3 b) { return c3(a.r * b.r, a.g * b.g
 template <class Ptr>
 __device__ __forceinline__ C3 c3p(Ptr p) { return c3(p[0], p[1], p[2]); }
 
+// b / 255.f for a byte b, the correctly rounded quotient: the product with
+// RN(1/255) and one fma correction, which equal the IEEE division for every
+// one of the 256 bytes (checked exhaustively on the host), in 3 instructions
+// where the division takes 11.
+__device__ __forceinline__ float byte_div255(uint32_t b) {
+    const float r = 0x1.010102p-8f;   // RN(1/255)
+    float x = (float)b;
+    float q = x * r;
+    float e = fmaf(-255.0f, q, x);
+    return fmaf(e, r, q);
+}
+
 // ---- textures: RGBA8 words holding TGAColor bytes b,g,r,a (SR/tgaimage.h:46-53) ----
 struct Tex {
     const uint32_t* texels;
@@ -146,15 +158,15 @@ __device__ __forceinline__ uint32_t tex_fetch(const Tex& t, float u, float v) {
 // Color(TGAColor) (SR/color.cpp:5) for IShader::tex_diffuse (SR/IShader.h:85-89)
 __device__ __forceinline__ C3 tex_diffuse(const Tex& t, float u, float v) {
     uint32_t c = tex_fetch(t, u, v);
-    return c3((float)((c >> 16) & 0xFF) / 255.f, (float)((c >> 8) & 0xFF) / 255.f, (float)(c & 0xFF) / 255.f);
+    return c3(byte_div255((c >> 16) & 0xFF), byte_div255((c >> 8) & 0xFF), byte_div255(c & 0xFF));
 }
 // IShader::tex_normal: res[2-i] = c[i]/255*2-1 (SR/IShader.h:91-99)
 __device__ __forceinline__ F3 tex_normal(const Tex& t, float u, float v) {
     uint32_t c = tex_fetch(t, u, v);
     F3 r;
-    r.z = (float)(c & 0xFF) / 255.f * 2.f - 1.f;
-    r.y = (float)((c >> 8) & 0xFF) / 255.f * 2.f - 1.f;
-    r.x = (float)((c >> 16) & 0xFF) / 255.f * 2.f - 1.f;
+    r.z = byte_div255(c & 0xFF) * 2.f - 1.f;
+    r.y = byte_div255((c >> 8) & 0xFF) * 2.f - 1.f;
+    r.x = byte_div255((c >> 16) & 0xFF) * 2.f - 1.f;
     return r;
 }
 
@@ -230,7 +242,7 @@ __device__ __forceinline__ int is_in_shadow(Uniforms u, const ShadowMap& sm, con
         size_t texel = (size_t)py * (size_t)sm.w + (size_t)px;
         float closest_depth;
         if (u.shadow_fp32) closest_depth = 2.f * gld(sm.depth + texel) - 1.f;
-        else closest_depth = (float)(gld(sm.color + texel) & 0xFF) / 255.f;
+        else closest_depth = byte_div255(gld(sm.color + texel) & 0xFF);
         return current_depth < closest_depth;
     }
     return 1;

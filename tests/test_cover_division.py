@@ -10,7 +10,10 @@ steps do nothing, which the kernel ensures by taking the sequence only for
 bit-identical either way; this pins the arithmetic itself on the host: for
 every reciprocal within 1 ulp (the correctly rounded one and its two
 neighbours) the sequence returns the correctly rounded quotient, over random
-and structured operands of that range.  Host C (gcc), fmaf from libm, no GPU."""
+and structured operands of that range.  The shading's byte / 255 (texels,
+the R8 shadow map) is one product with RN(1/255) and one fma correction;
+that equals the division for every byte.  Host C (gcc), fmaf from libm, no
+GPU."""
 import os
 import shutil
 import subprocess
@@ -103,3 +106,34 @@ def test_refined_reciprocal_quotients_are_correctly_rounded(tmp_path):
     out = subprocess.run([str(exe), "3000000"], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout[-2000:]
     assert "bad 0" in out.stdout
+
+
+BYTE_SRC = r"""
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+int main(void) {
+    const float r = 0x1.010102p-8f;
+    int bad = 0;
+    if (r != 1.0f / 255.0f) { printf("constant is not RN(1/255)\n"); return 1; }
+    for (int b = 0; b < 256; b++) {
+        float x = (float)b, q = x * r, e = fmaf(-255.0f, q, x), d = fmaf(e, r, q), ex = x / 255.0f;
+        uint32_t u, v;
+        memcpy(&u, &d, 4);
+        memcpy(&v, &ex, 4);
+        if (u != v) { printf("byte %d: %a against %a\n", b, d, ex); bad++; }
+    }
+    printf("bad %d\n", bad);
+    return bad != 0;
+}
+"""
+
+
+def test_byte_over_255_is_the_division(tmp_path):
+    c = tmp_path / "byte_div.c"
+    c.write_text(BYTE_SRC)
+    exe = tmp_path / "byte_div"
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-o", str(exe), str(c), "-lm"], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0 and "bad 0" in out.stdout, out.stdout
