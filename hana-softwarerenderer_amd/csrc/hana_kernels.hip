@@ -1176,6 +1176,13 @@ __device__ void sweep(TileLds<TS, RB>& L, const PassDesc& P, const Workspace& ws
     }
 }
 
+// Eight varyings of one corner (a 32-byte aligned row) in two 16-byte loads.
+__device__ __forceinline__ void load_varyings8(const float* row, float* y) {
+    float4 lo = gld4(reinterpret_cast<const float4*>(row)), hi = gld4(reinterpret_cast<const float4*>(row) + 1);
+    y[0] = lo.x; y[1] = lo.y; y[2] = lo.z; y[3] = lo.w;
+    y[4] = hi.x; y[5] = hi.y; y[6] = hi.z; y[7] = hi.w;
+}
+
 // Interpolates, shades and writes the winning fragment of one pixel
 // (interpolate_varyings + fragment + set_depth/set_color, SR/graphics.cpp:205-220, 362-373).
 __device__ __forceinline__ void shade_pixel(const PassDesc& P, const Workspace& ws, uint32_t order, int px, int py) {
@@ -1198,17 +1205,29 @@ __device__ __forceinline__ void shade_pixel(const PassDesc& P, const Workspace& 
         sb = ws.pool_vs + (base + j + 1) * POLY_VARY;
         sc = ws.pool_vs + (base + j + 2) * POLY_VARY;
     }
+    // the corners' live varyings, all loaded before any is awaited (8 per
+    // corner as two 16-byte loads: their rows are 32-byte aligned)
+    int cnt = varyings_of(P.kind);
+    float ya[8], yb[8], yc[8];
+    if (cnt == 8) {
+        load_varyings8(sa, ya);
+        load_varyings8(sb, yb);
+        load_varyings8(sc, yc);
+    } else {
+        for (int c = 0; c < 8; c++) ya[c] = yb[c] = yc[c] = 0.0f;
+        ya[0] = gld(sa); yb[0] = gld(sb); yc[0] = gld(sc);
+        if (cnt == 2) { ya[1] = gld(sa + 1); yb[1] = gld(sb + 1); yc[1] = gld(sc + 1); }
+    }
     TriS t;
     tri_setup(va, vb, vc, P.width, P.height, t);
     float wx, wy, wz, z;
     tri_cover(t, px, py, wx, wy, wz, z);
     float k0 = va.w * wx, k1 = vb.w * wy, k2 = vc.w * wz;
     float normalizer = 1 / (k0 + k1 + k2);
-    int cnt = varyings_of(P.kind);
     float vy[8];
     for (int c = 0; c < 8; c++) {
         if (c < cnt) {
-            float sum = gld(sa + c) * k0 + gld(sb + c) * k1 + gld(sc + c) * k2;
+            float sum = ya[c] * k0 + yb[c] * k1 + yc[c] * k2;
             vy[c] = sum * normalizer;
         } else {
             vy[c] = 0.0f;
