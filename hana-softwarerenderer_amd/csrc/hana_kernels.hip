@@ -409,7 +409,7 @@ __device__ __forceinline__ void wave_count(uint32_t* counters, int t, bool want)
         int lt = __shfl(t, leader);
         bool match = pending && t == lt;
         uint64_t same = __ballot(match);
-        if (lane == leader) atomicAdd(&counters[lt], (uint32_t)__popcll(same));
+        if (lane == leader) gadd(counters + lt, (uint32_t)__popcll(same));
         if (match) pending = false;
         mask = __ballot(pending);
     }
@@ -447,7 +447,7 @@ template <int BITS>
 __device__ __forceinline__ void lds_count(uint32_t* keys, uint32_t* cnt, uint32_t* global, uint32_t gt) {
     int sl = bin_slot<BITS>(keys, gt, true);
     if (sl >= 0) atomicAdd(&cnt[sl], 1u);
-    else atomicAdd(&global[gt], 1u);
+    else gadd(global + gt, 1u);
 }
 
 // Triangles over more tiles than this are counted and binned by a whole wave
@@ -478,8 +478,8 @@ __device__ __forceinline__ void count_tiles(const PassDesc& P, const Workspace& 
         ty1 = t.y1 >> TS;
         nx = tx1 - tx0 + 1;
         n = nx * (ty1 - ty0 + 1);
-        ws.tris[slot] = make_uint4(entry, (uint32_t)pass, (uint32_t)tx0 | ((uint32_t)ty0 << 16),
-                                   (uint32_t)tx1 | ((uint32_t)ty1 << 16));
+        gst4(ws.tris + slot, make_uint4(entry, (uint32_t)pass, (uint32_t)tx0 | ((uint32_t)ty0 << 16),
+                                        (uint32_t)tx1 | ((uint32_t)ty1 << 16)));
     }
     // a triangle over more than WIDE_TILES tiles is counted by the whole wave
     int tile0 = P.tile0, tiles_x = P.tiles_x;
@@ -493,7 +493,7 @@ __device__ __forceinline__ void count_tiles(const PassDesc& P, const Workspace& 
         int wn = __shfl(n, src), wnx = __shfl(nx, src), wx0 = __shfl(tx0, src), wy0 = __shfl(ty0, src);
         int wt0 = __shfl(tile0, src), wtx = __shfl(tiles_x, src);
         for (int k = __lane_id(); k < wn; k += 64)
-            atomicAdd(&ws.tile_count[wt0 + (wy0 + k / wnx) * wtx + wx0 + k % wnx], 1u);
+            gadd(ws.tile_count + (wt0 + (wy0 + k / wnx) * wtx + wx0 + k % wnx), 1u);
     }
     int ns = n > WIDE_TILES ? 0 : n;
     if constexpr (BITS == 0) {
@@ -541,8 +541,11 @@ __device__ __forceinline__ bool dies_at_first_crossed_plane(uint32_t c0, uint32_
 // (BLOCK * FPT records from blockIdx.x * BLOCK * FPT; the count goes to
 // ws.seg_count); faces that clipping would remove are dropped; the others are
 // queued for k_clip with one global atomic per workgroup.
+// Six waves per SIMD: the gathers of all FPT faces in flight take about 80
+// VGPRs; at 64 (eight waves) the compiler serialises them and the kernel is
+// slower, at 97 (five) it is slower too.
 template <int BITS, int TS>
-__global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* __restrict__ passes, int npasses, Workspace ws) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void k_setup(const PassDesc* __restrict__ passes, int npasses, Workspace ws) {
     constexpr int SLOTS = 1 << BITS;
     __shared__ uint64_t s_queue[BLOCK * FPT];
     __shared__ uint32_t s_keys[SLOTS], s_cnt[SLOTS];   // tile counts of the workgroup's triangles
@@ -559,32 +562,34 @@ __global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* __restrict__ pa
     const PassDesc& P = passes[pass];
     int base = (blockIdx.x - P.fblock0) * BLOCK * FPT;
     uint32_t seg = (uint32_t)blockIdx.x * (BLOCK * FPT);
+    // Lanes past the pass's last face load that face (a pass that owns a
+    // workgroup has one), so that no load waits on a branch: the indices of
+    // all FPT faces are in flight together, then the outcodes and
+    // post-transform vertices of all of them (gathered together, so the
+    // vertices do not wait for the trivial-accept test).
     int ids[FPT][3];
     bool valid[FPT];
 #pragma unroll
     for (int k = 0; k < FPT; k++) {
         int f = base + k * BLOCK + threadIdx.x;
         valid[k] = f < P.nfaces;
-        ids[k][0] = ids[k][1] = ids[k][2] = 0;
-        if (valid[k]) {
-            ids[k][0] = P.idx[3 * f];
-            ids[k][1] = P.idx[3 * f + 1];
-            ids[k][2] = P.idx[3 * f + 2];
-        }
+        int fl = valid[k] ? f : P.nfaces - 1;
+        ids[k][0] = gld(P.idx + 3 * fl);
+        ids[k][1] = gld(P.idx + 3 * fl + 1);
+        ids[k][2] = gld(P.idx + 3 * fl + 2);
     }
-    // outcodes and post-transform vertices are gathered together (every
-    // vertex has both), so the vertices do not wait for the trivial-accept test
     uint32_t codes[FPT][3];
     float4 va[FPT], vb[FPT], vc[FPT];
 #pragma unroll
     for (int k = 0; k < FPT; k++) {
-        for (int j = 0; j < 3; j++) codes[k][j] = valid[k] ? P.code[ids[k][j]] : 0u;
-        if (valid[k]) {
-            va[k] = P.vr[ids[k][0]];
-            vb[k] = P.vr[ids[k][1]];
-            vc[k] = P.vr[ids[k][2]];
-        }
+        for (int j = 0; j < 3; j++) codes[k][j] = gld(P.code + ids[k][j]);
+        va[k] = gld4(P.vr + ids[k][0]);
+        vb[k] = gld4(P.vr + ids[k][1]);
+        vc[k] = gld4(P.vr + ids[k][2]);
     }
+#pragma unroll
+    for (int k = 0; k < FPT; k++)
+        for (int j = 0; j < 3; j++) codes[k][j] = valid[k] ? codes[k][j] : 0u;
     bool trivial[FPT];
 #pragma unroll
     for (int k = 0; k < FPT; k++) trivial[k] = valid[k] && (codes[k][0] & codes[k][1] & codes[k][2] & CODE_VISIBLE);
@@ -593,7 +598,7 @@ __global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* __restrict__ pa
         int f = base + k * BLOCK + threadIdx.x;
         bool clipped = valid[k] && !trivial[k];
         bool clipq = clipped && !dies_at_first_crossed_plane(codes[k][0], codes[k][1], codes[k][2]);
-        if (clipped && !clipq) P.poly_of_face[f] = -1;
+        if (clipped && !clipq) gst(P.poly_of_face + f, -1);
         uint32_t q = wave_reserve_lds(&s_nq, clipq);
         if (clipq) s_queue[q] = ((uint64_t)(uint32_t)pass << 32) | (uint32_t)f;
         if (P.stats) {
@@ -614,20 +619,20 @@ __global__ __launch_bounds__(BLOCK) void k_setup(const PassDesc* __restrict__ pa
     uint2* list = ws.seg_tiles + (size_t)blockIdx.x * SLOTS;
     for (int i = threadIdx.x; i < SLOTS; i += BLOCK)
         if (s_keys[i] != BIN_EMPTY) {
-            atomicAdd(&ws.tile_count[s_keys[i]], s_cnt[i]);
-            list[atomicAdd(&s_nlist, 1u)] = make_uint2(s_keys[i], s_cnt[i]);
+            gadd(ws.tile_count + s_keys[i], s_cnt[i]);
+            gst2(list + atomicAdd(&s_nlist, 1u), make_uint2(s_keys[i], s_cnt[i]));
         }
     uint32_t nwide = s_nwide;
-    for (uint32_t i = threadIdx.x; i < nwide; i += BLOCK) ws.seg_wide[(size_t)blockIdx.x * (BLOCK * FPT) + i] = s_wide[i];
+    for (uint32_t i = threadIdx.x; i < nwide; i += BLOCK) gst(ws.seg_wide + (size_t)blockIdx.x * (BLOCK * FPT) + i, s_wide[i]);
     if (threadIdx.x == 0) {
-        ws.seg_count[blockIdx.x] = s_ntri;
+        gst(ws.seg_count + blockIdx.x, s_ntri);
         s_qbase = s_nq ? atomicAdd(&ws.status[STATUS_CLIPQ], s_nq) : 0u;
     }
     __syncthreads();
-    if (threadIdx.x == 0) ws.seg_info[blockIdx.x] = s_nlist | (nwide << 16);
+    if (threadIdx.x == 0) gst(ws.seg_info + blockIdx.x, s_nlist | (nwide << 16));
     for (uint32_t i = threadIdx.x; i < s_nq; i += BLOCK) {
         uint32_t q = s_qbase + i;
-        if (q < ws.clipq_cap) ws.clip_queue[q] = s_queue[i];
+        if (q < ws.clipq_cap) gst(ws.clip_queue + q, s_queue[i]);
         else raise_flag(ws, FLAG_CLIPQ_OVERFLOW);
     }
 }
