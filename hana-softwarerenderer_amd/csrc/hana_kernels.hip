@@ -450,6 +450,15 @@ __device__ __forceinline__ void lds_count(uint32_t* keys, uint32_t* cnt, uint32_
     else gadd(global + gt, 1u);
 }
 
+// A workgroup barrier that orders LDS accesses only: unlike __syncthreads it
+// does not wait for the wave's outstanding global loads first, so loads
+// issued before it stay in flight across it.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Triangles over more tiles than this are counted and binned by a whole wave
 // (or workgroup), their tiles spread over the lanes.
 constexpr int WIDE_TILES = 16;
@@ -553,11 +562,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
     __shared__ uint32_t s_ntri, s_nq, s_qbase, s_nwide, s_nlist;
     if (threadIdx.x == 0) { s_ntri = 0; s_nq = 0; s_nwide = 0; s_nlist = 0; }
     const SegTables L{s_keys, s_cnt, s_wide, &s_nwide};
-    for (int i = threadIdx.x; i < SLOTS; i += BLOCK) {
-        s_keys[i] = BIN_EMPTY;
-        s_cnt[i] = 0u;
-    }
-    __syncthreads();
     int pass = find_pass<1>(passes, npasses, blockIdx.x);
     const PassDesc& P = passes[pass];
     int base = (blockIdx.x - P.fblock0) * BLOCK * FPT;
@@ -566,7 +570,9 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
     // workgroup has one), so that no load waits on a branch: the indices of
     // all FPT faces are in flight together, then the outcodes and
     // post-transform vertices of all of them (gathered together, so the
-    // vertices do not wait for the trivial-accept test).
+    // vertices do not wait for the trivial-accept test).  They are issued
+    // before the tile table is cleared, so that the clear and its barrier
+    // overlap them.
     int ids[FPT][3];
     bool valid[FPT];
 #pragma unroll
@@ -587,6 +593,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(6))) void
         vb[k] = gld4(P.vr + ids[k][1]);
         vc[k] = gld4(P.vr + ids[k][2]);
     }
+    for (int i = threadIdx.x; i < SLOTS; i += BLOCK) {
+        s_keys[i] = BIN_EMPTY;
+        s_cnt[i] = 0u;
+    }
+    lds_barrier();
 #pragma unroll
     for (int k = 0; k < FPT; k++)
         for (int j = 0; j < 3; j++) codes[k][j] = valid[k] ? codes[k][j] : 0u;
@@ -813,14 +824,19 @@ __global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* __restrict__ pass
     if ((int)blockIdx.x < nseg) {
         __shared__ uint32_t keys[BIN_HASH];
         __shared__ uint32_t cnt[BIN_HASH];   // a slot's next bin position, set when its tile is inserted
+        // the segment's sizes (written by k_setup, uniform: scalar loads) are
+        // read before the table is cleared, so that they are not awaited after it
+        const uint32_t n = *(const HANA_CONST_AS uint32_t*)(ws.seg_count + blockIdx.x);
+        const uint32_t info = *(const HANA_CONST_AS uint32_t*)(ws.seg_info + blockIdx.x);
+        const uint4* rec = ws.tris + (size_t)blockIdx.x * (BLOCK * FPT);
+        // a segment's triangles are all of one pass (one k_setup workgroup's faces)
+        const uint32_t spass = n ? *(const HANA_CONST_AS uint32_t*)&rec[0].y : 0u;
         for (int i = threadIdx.x; i < BIN_HASH; i += BLOCK) keys[i] = BIN_EMPTY;
         __syncthreads();
-        uint32_t n = ws.seg_count[blockIdx.x];
-        const uint4* rec = ws.tris + (size_t)blockIdx.x * (BLOCK * FPT);
+        const PassDesc& P = passes[spass];
         // the segment's distinct tiles and their pair counts, as k_setup listed
         // them (a tile that found no slot there, or finds none here, goes
         // through the global cursor pair by pair below)
-        uint32_t info = ws.seg_info[blockIdx.x];
         uint32_t nlist = info & 0xFFFFu, nw = info >> 16;
         const uint2* list = ws.seg_tiles + (size_t)blockIdx.x * ws.seg_stride;
         // listed tiles are distinct, so an inserted slot is this thread's:
@@ -833,7 +849,6 @@ __global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* __restrict__ pass
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
             uint4 r = gld4(rec + i);
-            const PassDesc& P = passes[r.y];
             int tx0 = (int)(r.z & 0xFFFFu), ty0 = (int)(r.z >> 16);
             int tx1 = (int)(r.w & 0xFFFFu), ty1 = (int)(r.w >> 16);
             if ((tx1 - tx0 + 1) * (ty1 - ty0 + 1) > WIDE_TILES) continue;
@@ -849,7 +864,6 @@ __global__ __launch_bounds__(BLOCK) void k_bin(const PassDesc* __restrict__ pass
         const uint16_t* wide = ws.seg_wide + (size_t)blockIdx.x * (BLOCK * FPT);
         for (uint32_t w = threadIdx.x >> 6; w < nw; w += BLOCK / 64) {
             uint4 r = gld4(rec + gld(wide + w));
-            const PassDesc& P = passes[r.y];
             int tx0 = (int)(r.z & 0xFFFFu), ty0 = (int)(r.z >> 16);
             int nx = (int)(r.w & 0xFFFFu) - tx0 + 1;
             int nt = nx * ((int)(r.w >> 16) - ty0 + 1);
