@@ -675,19 +675,19 @@ __global__ __launch_bounds__(CLIP_THREADS) void k_clip(const PassDesc* __restric
         bool have = q < count;                       // uniform within the group
         int pass = 0, f = 0;
         if (have) {
-            uint64_t e = ws.clip_queue[q];
+            uint64_t e = gld(ws.clip_queue + q);
             pass = (int)(e >> 32);
             f = (int)(uint32_t)e;
         }
         const PassDesc& P = passes[pass];
         if (have && li < 3) {
-            int v = P.idx[3 * f + li];
+            int v = gld(P.idx + 3 * f + li);
             bool need_n = P.kind != SHADER_SHADOW && P.kind != SHADER_TEXTURE;
-            F3 p = f3(P.px[v], P.py[v], P.pz[v]);
-            F3 nn = need_n ? f3(P.nx[v], P.ny[v], P.nz[v]) : f3(0.f, 0.f, 0.f);
+            F3 p = f3(gld(P.px + v), gld(P.py + v), gld(P.pz + v));
+            F3 nn = need_n ? f3(gld(P.nx + v), gld(P.ny + v), gld(P.nz + v)) : f3(0.f, 0.f, 0.f);
             float clip[4], vary[8];
             for (int c = 0; c < 8; c++) vary[c] = 0.0f;
-            vertex_program(P.kind, uniforms_of(P.u), p, nn, P.tu[v], P.tv[v], clip, vary);
+            vertex_program(P.kind, uniforms_of(P.u), p, nn, gld(P.tu + v), gld(P.tv + v), clip, vary);
             for (int c = 0; c < 4; c++) buf[g][0][li][c] = clip[c];
             for (int c = 0; c < POLY_VARY; c++) buf[g][0][li][4 + c] = vary[c];
         }
@@ -759,19 +759,19 @@ __global__ __launch_bounds__(CLIP_THREADS) void k_clip(const PassDesc* __restric
             ok = reserved = false;
         }
         // a reserved record of a dropped polygon is written empty (no tiles) for k_bin
-        if (reserved && !ok) ws.tris[ws.seg_region + slot] = make_uint4(0u, 0u, 1u, 0u);
+        if (reserved && !ok) gst4(ws.tris + (ws.seg_region + slot), make_uint4(0u, 0u, 1u, 0u));
         if (keep && li < n) {
-            ws.pool_vr[(size_t)pid * POLY_MAX + li] = ndc_of(poly[li]);
-            for (int c = 0; c < POLY_VARY; c++) ws.pool_vs[((size_t)pid * POLY_MAX + li) * POLY_VARY + c] = poly[li][4 + c];
+            gst4(ws.pool_vr + ((size_t)pid * POLY_MAX + li), ndc_of(poly[li]));
+            for (int c = 0; c < POLY_VARY; c++) gst(ws.pool_vs + (((size_t)pid * POLY_MAX + li) * POLY_VARY + c), poly[li][4 + c]);
         }
         if (have && li == 0) {
             if (keep) {
-                ws.pool_face[pid] = f;
-                ws.pool_pass[pid] = pass;
-                ws.pool_n[pid] = n;
+                gst(ws.pool_face + pid, f);
+                gst(ws.pool_pass + pid, pass);
+                gst(ws.pool_n + pid, n);
                 stat_add(P, ST_POLYS, 1);
             }
-            P.poly_of_face[f] = keep ? (int)pid : -1;
+            gst(P.poly_of_face + f, keep ? (int)pid : -1);
         }
         count_tiles<0, TS>(P, ws, pass, ok, t, CLIPPED_BIT | (pid << FAN_BITS) | (uint32_t)li, ws.seg_region + slot,
                            0u, nullptr);
