@@ -184,33 +184,6 @@ __device__ __forceinline__ bool tri_cover_fast(const TriS& t, int px, int py, fl
     return true;
 }
 
-// A division-free test that rejects only pixels tri_cover rejects too (it
-// never rejects a covered one), so that the three divisions run only where a
-// fragment can result.  With |u.z| <= 2^60:
-//   w.y = fl(u.y/u.z) < 0 for sure if u.y and u.z differ in sign and
-//     |u.y| >= 2^-60 (the quotient is then at least 2^-120 in magnitude and
-//     cannot round to -0); w.z likewise with u.x;
-//   w.x = fl(1 - fl(s/u.z)) < 0, s = fl(u.x + u.y), for sure if
-//     s/u.z > 1 + 2^-17: tested as s > fl(u.z*(1+2^-16)) for u.z > 0 (or <
-//     for u.z < 0); fl(u.z*(1+2^-16)) errs by at most 2^-24 relative, so the
-//     quotient exceeds 1 + 2^-17 and rounds above 1.
-// NaN operands fail every comparison and are left to tri_cover.
-__device__ __forceinline__ bool maybe_covered(const TriS& t, int px, int py) {
-    float Px = (float)px, Py = (float)py;
-    float s0z = t.ax - Px;
-    float s1z = t.ay - Py;
-    float ux = t.s0y * s1z - s0z * t.s1y;
-    float uy = s0z * t.s1x - t.s0x * s1z;
-    if (!(fabsf(t.uz) <= 0x1p60f)) return true;
-    uint32_t sz = __float_as_uint(t.uz) & 0x80000000u;
-    if (fabsf(uy) >= 0x1p-60f && (__float_as_uint(uy) & 0x80000000u) != sz) return false;
-    if (fabsf(ux) >= 0x1p-60f && (__float_as_uint(ux) & 0x80000000u) != sz) return false;
-    float s = ux + uy;
-    float lim = t.uz * 1.0000152587890625f;   // 1 + 2^-16
-    if (sz ? (s < lim) : (s > lim)) return false;
-    return true;
-}
-
 // order-preserving 32-bit image of a depth (-0 folded onto +0)
 __device__ __forceinline__ uint32_t depth_bits(float z) {
     uint32_t u = __float_as_uint(z);
@@ -1102,15 +1075,14 @@ __device__ __forceinline__ TriS staged_tri(const TileLds<TS, RB>& L, int s, uint
 }
 
 // One sweep over the tile's bin: small triangles one per lane (in passes that
-// share them, those of more than HANA_MED_PIX candidates outside the reject
-// pass HANA_MED_LANES lanes each), larger ones one per wave with the lanes
-// spread over their candidate pixels.
+// share them, those of more than HANA_MED_PIX candidates HANA_MED_LANES lanes
+// each), larger ones one per wave with the lanes spread over their candidate
+// pixels.
 template <int TS, int RB>
 __device__ void sweep(TileLds<TS, RB>& L, const PassDesc& P, const Workspace& ws, uint32_t beg, uint32_t cnt, int xb, int yb,
                       bool second) {
     constexpr int EDGE = 1 << TS;
     int tid = threadIdx.x, lane = tid & 63;
-    bool er = cnt > (uint32_t)HANA_ER_MIN_ENTRIES;   // uniform over the workgroup
     for (uint32_t base = 0; base < cnt; base += RB) {
         bool ok = false;
         TriS t;
@@ -1126,24 +1098,9 @@ __device__ void sweep(TileLds<TS, RB>& L, const PassDesc& P, const Workspace& ws
         if (ok) {
             int wdt = bx1 - bx0 + 1;
             int npix = wdt * (by1 - by0 + 1);
-            if (npix <= SMALL_PIX && (!P.share_small || npix <= HANA_MED_PIX || (er && npix <= HANA_ER_MAX_PIX))) {
-                if (er && npix <= HANA_ER_MAX_PIX) {
-                    // cheap pass over the candidates, then the exact test on the survivors
-                    uint32_t mask = 0;
-                    int k = 0;
-                    for (int ly = by0; ly <= by1; ly++)
-                        for (int lx = bx0; lx <= bx1; lx++, k++)
-                            if (maybe_covered(t, xb + lx, yb + ly)) mask |= 1u << k;
-                    uint32_t magic = tile_div_magic(wdt);
-                    while (mask) {
-                        int b = __builtin_ctz(mask);
-                        mask &= mask - 1u;
-                        int row = tile_div(b, magic);
-                        frag_resolve<TS, RB>(L, t, lo, xb, yb, bx0 + b - row * wdt, by0 + row, second);
-                    }
-                } else
-                    for (int ly = by0; ly <= by1; ly++)
-                        for (int lx = bx0; lx <= bx1; lx++) frag_resolve<TS, RB>(L, t, lo, xb, yb, lx, ly, second);
+            if (npix <= SMALL_PIX && (!P.share_small || npix <= HANA_MED_PIX)) {
+                for (int ly = by0; ly <= by1; ly++)
+                    for (int lx = bx0; lx <= bx1; lx++) frag_resolve<TS, RB>(L, t, lo, xb, yb, lx, ly, second);
             } else {
                 L.ax[tid] = t.ax; L.ay[tid] = t.ay; L.s0x[tid] = t.s0x; L.s0y[tid] = t.s0y;
                 L.s1x[tid] = t.s1x; L.s1y[tid] = t.s1y; L.uz[tid] = t.uz;

@@ -25,20 +25,10 @@ constexpr int BLOCK = 256;                 // threads per workgroup (4 wave64)
 #ifndef HANA_SMALL_PIX
 #define HANA_SMALL_PIX 32
 #endif
-// The division-free reject pass runs for small triangles of up to
-// HANA_ER_MAX_PIX candidates, in tiles whose bin holds more than
-// HANA_ER_MIN_ENTRIES triangles (dense micro-triangle tiles, where many
-// candidates fail; in sparse tiles the extra pass costs more than it saves).
-#ifndef HANA_ER_MAX_PIX
-#define HANA_ER_MAX_PIX 32
-#endif
-#ifndef HANA_ER_MIN_ENTRIES
-#define HANA_ER_MIN_ENTRIES 320
-#endif
 // In passes of over 4 target pixels per face, small triangles of more than
-// HANA_MED_PIX candidates (outside the reject pass) are shared by
-// HANA_MED_LANES lanes each instead of looped over by one: a wave's trip count
-// is then bounded by HANA_SMALL_PIX / HANA_MED_LANES.  Passes of smaller faces
+// HANA_MED_PIX candidates are shared by HANA_MED_LANES lanes each instead of
+// looped over by one: a wave's trip count is then bounded by HANA_SMALL_PIX /
+// HANA_MED_LANES.  Passes of smaller faces
 // (micro-triangle meshes) keep one lane per triangle, which their few
 // candidates suit better.
 #ifndef HANA_MED_PIX
@@ -49,7 +39,7 @@ constexpr int BLOCK = 256;                 // threads per workgroup (4 wave64)
 #endif
 // A raster workgroup (one tile at a time) has 2 or 4 waves: 128 or 256 lanes,
 // at most one lane per pixel of a 16x16 tile.
-static_assert(HANA_SMALL_PIX >= 1 && HANA_SMALL_PIX <= 32, "small-triangle candidates fit a 32-bit mask");
+static_assert(HANA_SMALL_PIX >= 1 && HANA_SMALL_PIX <= 32, "small-triangle candidates: at most 32");
 #ifndef HANA_VPT
 #define HANA_VPT 4
 #endif
