@@ -1015,7 +1015,10 @@ struct TileLds {
     float z0[RB], z1[RB], z2[RB];
     uint32_t lo[RB], box[RB];
     int big[RB], med[RB];
-    int nbig, head, nmed, mhead, nan_seen;
+    // staging counters of sweep round r in set r % 3 (nbig, head, nmed, mhead):
+    // round r clears set (r + 2) % 3 once every wave is past round r - 1
+    int cnt[3][4];
+    int nan_seen;
 };
 
 __device__ __forceinline__ bool load_tri(const PassDesc& P, const Workspace& ws, uint32_t e, TriS& t, uint32_t& order) {
@@ -1079,11 +1082,12 @@ __device__ __forceinline__ TriS staged_tri(const TileLds<TS, RB>& L, int s, uint
 // each), larger ones one per wave with the lanes spread over their candidate
 // pixels.
 template <int TS, int RB>
-__device__ void sweep(TileLds<TS, RB>& L, const PassDesc& P, const Workspace& ws, uint32_t beg, uint32_t cnt, int xb, int yb,
-                      bool second) {
+__device__ int sweep(TileLds<TS, RB>& L, const PassDesc& P, const Workspace& ws, uint32_t beg, uint32_t cnt, int xb, int yb,
+                     bool second, int set) {
     constexpr int EDGE = 1 << TS;
     int tid = threadIdx.x, lane = tid & 63;
     for (uint32_t base = 0; base < cnt; base += RB) {
+        int* C = L.cnt[set];
         bool ok = false;
         TriS t;
         uint32_t order = 0;
@@ -1107,18 +1111,24 @@ __device__ void sweep(TileLds<TS, RB>& L, const PassDesc& P, const Workspace& ws
                 L.z0[tid] = t.z0; L.z1[tid] = t.z1; L.z2[tid] = t.z2;
                 L.lo[tid] = lo;
                 L.box[tid] = (uint32_t)bx0 | ((uint32_t)bx1 << 8) | ((uint32_t)by0 << 16) | ((uint32_t)by1 << 24);
-                if (npix <= SMALL_PIX) L.med[atomicAdd(&L.nmed, 1)] = tid;
-                else L.big[atomicAdd(&L.nbig, 1)] = tid;
+                if (npix <= SMALL_PIX) L.med[atomicAdd(&C[2], 1)] = tid;
+                else L.big[atomicAdd(&C[0], 1)] = tid;
             }
         }
         __syncthreads();
+        // every wave is past round r - 1: its set is cleared for round r + 2
+        const int nmed = C[2], nbig = C[0];
+        if (tid == 0) {
+            int* R = L.cnt[set == 0 ? 2 : set - 1];
+            R[0] = 0; R[1] = 0; R[2] = 0; R[3] = 0;
+        }
         // shared small triangles: 64 / MED_LANES per wave and round
         for (;;) {
             constexpr int PER_WAVE = 64 / HANA_MED_LANES;
+            if (nmed == 0) break;
             int q = 0;
-            if (lane == 0) q = atomicAdd(&L.mhead, PER_WAVE);
+            if (lane == 0) q = atomicAdd(&C[3], PER_WAVE);
             q = __shfl(q, 0);
-            int nmed = L.nmed;
             if (q >= nmed) break;
             int m = q + lane / HANA_MED_LANES;
             if (m < nmed) {
@@ -1133,10 +1143,11 @@ __device__ void sweep(TileLds<TS, RB>& L, const PassDesc& P, const Workspace& ws
             }
         }
         for (;;) {
+            if (nbig == 0) break;
             int q = 0;
-            if (lane == 0) q = atomicAdd(&L.head, 1);
+            if (lane == 0) q = atomicAdd(&C[1], 1);
             q = __shfl(q, 0);
-            if (q >= L.nbig) break;
+            if (q >= nbig) break;
             uint32_t blo;
             int x0, y0, wdt, npix;
             TriS bt = staged_tri<TS, RB>(L, L.big[q], blo, x0, y0, wdt, npix);
@@ -1146,10 +1157,12 @@ __device__ void sweep(TileLds<TS, RB>& L, const PassDesc& P, const Workspace& ws
                 frag_resolve<TS, RB>(L, bt, blo, xb, yb, x0 + k - row * wdt, y0 + row, second);
             }
         }
-        __syncthreads();
-        if (tid == 0) { L.nbig = 0; L.head = 0; L.nmed = 0; L.mhead = 0; }
-        __syncthreads();
+        // staged triangles are read until every wave is done with them (a
+        // round that staged none wrote all its fragments before the barrier above)
+        if (nmed | nbig) __syncthreads();
+        set = set == 2 ? 0 : set + 1;
     }
+    return set;
 }
 
 // Eight varyings of one corner (a 32-byte aligned row) in two 16-byte loads.
@@ -1281,10 +1294,11 @@ __device__ void raster_tile(TileLds<TS, RB>& L, const PassDesc& P, const Workspa
         L.key[i] = k;
         L.nan_order[i] = 0u;
     }
-    if (tid == 0) { L.nbig = 0; L.head = 0; L.nmed = 0; L.mhead = 0; L.nan_seen = 0; }
+    if (tid < 12) (&L.cnt[0][0])[tid] = 0;
+    if (tid == 0) L.nan_seen = 0;
     __syncthreads();
 
-    sweep<TS, RB>(L, P, ws, beg, cnt, xb, yb, false);
+    int set = sweep<TS, RB>(L, P, ws, beg, cnt, xb, yb, false, 0);
 
     if (L.nan_seen) {
         // pixels that saw a NaN-depth fragment restart after the last one
@@ -1293,7 +1307,7 @@ __device__ void raster_tile(TileLds<TS, RB>& L, const PassDesc& P, const Workspa
             if (no) L.key[i] = (0xFFFFFFFFull << 32) | (uint64_t)(0xFFFFFFFEu - (no - 1u));
         }
         __syncthreads();
-        sweep<TS, RB>(L, P, ws, beg, cnt, xb, yb, true);
+        sweep<TS, RB>(L, P, ws, beg, cnt, xb, yb, true, set);
     }
 
     uint32_t written = 0;
