@@ -203,6 +203,7 @@ struct hana_batch_part {
     std::vector<hana::rt::PassSpec> specs;
     hana::rt::Plan plan;
     hana::rt::Arena* arena = nullptr;   // the batch's arena of the same index
+    size_t f0 = 0, f1 = 0;               // its frames [f0, f1) of the batch
 };
 
 struct hana_batch {

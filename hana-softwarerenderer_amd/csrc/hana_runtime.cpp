@@ -458,6 +458,10 @@ int enqueue_raster(hana_ctx* c, const Plan& pl, const Arena& A, hipStream_t s, b
 }
 
 int enqueue_sequence(hana_ctx* c, const Plan& pl, const Arena& A, hipStream_t s, bool timed = true) {
+    // every kernel reads the workspace: a plan that was never bound to an
+    // arena must not reach the device
+    if (pl.descs.empty() || !pl.ws.status || !A.tables.p)
+        return fail(c, HANA_E_INVALID, "internal error: a sequence without a bound plan");
     int r = enqueue_geometry(c, pl, A, s, timed);
     return r ? r : enqueue_raster(c, pl, A, s, timed);
 }
@@ -1179,30 +1183,35 @@ static void batch_drop_pending(hana_batch* b) {
     for (auto& s : b->specs) s.target->pending_clear = false;
 }
 
-// Splits the frames into up to b->tuning.parts parts of about equal work
-// (faces plus an eighth of the pixels drawn), at most one part per frame.
-// Four parts measured best on 256 mixed scenes (2.10 ms per batch, against
-// 2.20 for two and 2.3-2.4 for six or eight): a process gets four hardware
-// queues, and more streams than that share them.
-static void batch_split(hana_batch* b) {
+// Each frame's first work estimate: faces plus an eighth of the pixels drawn,
+// over its passes.
+static std::vector<double> frame_weights(const hana_batch* b) {
     size_t nf = b->frame_first.size();
-    b->parts.clear();
-    size_t np = std::min(nf, std::min((size_t)b->tuning.parts, kMaxBatchParts));
     std::vector<double> w(nf, 0.0);
-    double total = 0.0;
     for (size_t k = 0; k < nf; k++) {
         size_t e = k + 1 < nf ? b->frame_first[k + 1] : b->specs.size();
         for (size_t i = b->frame_first[k]; i < e; i++)
             w[k] += (double)b->specs[i].mesh->F + (double)b->specs[i].target->w * b->specs[i].target->h / 8.0;
-        total += w[k];
     }
+    return w;
+}
+
+// The end frame of each part when the frames, in order, are split into up to
+// b->tuning.parts parts of about equal estimated work w (one value per frame),
+// at most one part per frame.  Four parts measured best on 256 mixed scenes
+// (2.10 ms per batch, against 2.20 for two and 2.3-2.4 for six or eight): a
+// process gets four hardware queues, and more streams than that share them.
+static std::vector<size_t> split_ends(const hana_batch* b, const std::vector<double>& w) {
+    size_t nf = b->frame_first.size();
+    size_t np = std::min(nf, std::min((size_t)b->tuning.parts, kMaxBatchParts));
+    double total = 0.0;
+    for (double x : w) total += x;
     // part p ends at the first frame where the running work reaches (p+1)/np
     // of the total, leaving at least one frame for each later part
-    b->parts.resize(np);
+    std::vector<size_t> ends(np);
     size_t k = 0;
     double run = 0.0;
     for (size_t p = 0; p < np; p++) {
-        size_t k0 = k;
         if (p + 1 == np) {
             k = nf;
         } else {
@@ -1210,9 +1219,25 @@ static void batch_split(hana_batch* b) {
                 run += w[k++];
             } while (k < nf - (np - 1 - p) && run < total * (double)(p + 1) / (double)np);
         }
+        ends[p] = k;
+    }
+    return ends;
+}
+
+// Replaces the batch's parts by those ending at `ends` (unsized: no plan yet).
+static void batch_split(hana_batch* b, const std::vector<size_t>& ends) {
+    size_t nf = b->frame_first.size();
+    b->parts.clear();
+    b->parts.resize(ends.size());
+    size_t k0 = 0;
+    for (size_t p = 0; p < ends.size(); p++) {
+        size_t k = ends[p];
         size_t s0 = b->frame_first[k0], s1 = k < nf ? b->frame_first[k] : b->specs.size();
         b->parts[p].specs.assign(b->specs.begin() + s0, b->specs.begin() + s1);
         b->parts[p].arena = &b->arenas[p];
+        b->parts[p].f0 = k0;
+        b->parts[p].f1 = k;
+        k0 = k;
     }
 }
 
@@ -1300,9 +1325,51 @@ static void batch_drop_graph(hana_batch* b) {
 static int batch_prepare(hana_ctx* c, hana_batch* b, bool use_graph) {
     int r;
     batch_drop_graph(b);
-    batch_split(b);
+    std::vector<double> w = frame_weights(b);
+    batch_split(b, split_ends(b, w));
     for (auto& part : b->parts)
         if ((r = batch_size_part(c, b, part))) return r;
+    if (b->parts.size() > 1) {
+        // One rebalancing step.  The parts run concurrently and the batch ends
+        // with its slowest one; with the first estimates the parts of 256
+        // mixed scenes ended up to 0.27 ms apart.  Each part is timed alone
+        // (a run to warm up, then a timed one: the sequence clears what it
+        // draws, so the runs leave nothing behind), its frames' estimates are
+        // scaled by its measured time per unit of estimate, and the frames
+        // are split again; if any part's frames changed, every part is sized again.
+        std::vector<double> cost(w.size(), 0.0);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIP_TRY(c, hipEventCreate(&e0));
+        if (hipEventCreate(&e1) != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            return fail(c, HANA_E_DEVICE, "hipEventCreate");
+        }
+        r = HANA_OK;
+        for (auto& part : b->parts) {
+            float ms = 0.0f;
+            for (int rep = 0; rep < 2 && !r; rep++) {
+                if (hipEventRecord(e0, c->stream) != hipSuccess) r = fail(c, HANA_E_DEVICE, "hipEventRecord");
+                if (!r) r = enqueue_sequence(c, part.plan, *part.arena, c->stream, false);
+                if (!r && hipEventRecord(e1, c->stream) != hipSuccess) r = fail(c, HANA_E_DEVICE, "hipEventRecord");
+                if (!r && hipEventSynchronize(e1) != hipSuccess) r = fail(c, HANA_E_DEVICE, "hipEventSynchronize");
+                if (!r && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) r = fail(c, HANA_E_DEVICE, "hipEventElapsedTime");
+            }
+            double W = 0.0;
+            for (size_t k = part.f0; k < part.f1; k++) W += w[k];
+            for (size_t k = part.f0; k < part.f1; k++) cost[k] = W > 0.0 ? w[k] * (double)ms / W : w[k];
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        batch_drop_pending(b);
+        if (r) return r;
+        std::vector<size_t> ends = split_ends(b, cost), before;
+        for (auto& part : b->parts) before.push_back(part.f1);
+        if (ends != before) {
+            batch_split(b, ends);
+            for (auto& part : b->parts)
+                if ((r = batch_size_part(c, b, part))) return r;
+        }
+    }
     if (b->parts.size() > 1 && !b->fork) HIP_TRY(c, hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
     while (b->joined.size() + 1 < b->parts.size()) {
         hipEvent_t e = nullptr;
