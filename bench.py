@@ -801,8 +801,8 @@ def main():
         print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4, "launches": 5}))
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}): every GPU box here has one GPU; "
-              "how the path would shard is described in DESIGN.md.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); run it with --gpus 1. Frames, batch "
+              "parts and server requests are independent, so several GPUs would each take their own.", file=sys.stderr)
         sys.exit(2)
     sections = set(args.sections.split(","))
 

@@ -1,5 +1,5 @@
-"""Parity helpers: the GPU result against the oracle's (bar of BASELINE.json:
-depth bit-exact, colour <= 1 LSB on >= 99.9 % of pixels and <= 2 LSB everywhere)."""
+"""Parity helpers: the GPU result against the oracle's.  The bar: depth
+bit-exact, colour <= 1 LSB on >= 99.9 % of pixels and <= 2 LSB everywhere."""
 import numpy as np
 
 
