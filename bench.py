@@ -741,15 +741,35 @@ def summarize_counters(workload, d, out_json):
               f"{'' if util is None else f'{util:.3f}'} | {o.get('valu_trans_f32', '')} |")
 
 
-def summarize_trace(d):
+def _kernel_name(raw):
+    return raw.replace("void ", "").split("(")[0].replace("hana::", "")
+
+
+def summarize_trace(d, last=0):
+    """The rocprofv3 --stats summary under d as a markdown table; with last > 0,
+    also each kernel's mean over its last `last` dispatches in start order."""
     rows = _csv_rows(d, "kernel_stats.csv")
     total = sum(float(r["TotalDurationNs"]) for r in rows) or 1.0
     print("| kernel | calls | average µs | min µs | max µs | share of kernel time |")
     print("|---|---:|---:|---:|---:|---:|")
     for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
-        name = r["Name"].replace("void ", "").split("(")[0].replace("hana::", "")
-        print(f"| `{name}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | {float(r['MinNs']) / 1e3:.2f} | "
-              f"{float(r['MaxNs']) / 1e3:.2f} | {100 * float(r['TotalDurationNs']) / total:.1f} % |")
+        print(f"| `{_kernel_name(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
+              f"{float(r['MinNs']) / 1e3:.2f} | {float(r['MaxNs']) / 1e3:.2f} | "
+              f"{100 * float(r['TotalDurationNs']) / total:.1f} % |")
+    if last <= 0:
+        return
+    by = {}
+    for r in sorted(_csv_rows(d, "kernel_trace.csv"), key=lambda r: int(r["Start_Timestamp"])):
+        by.setdefault(_kernel_name(r["Kernel_Name"]), []).append(
+            (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    print(f"\n## The last {last} dispatches of each kernel (kernel_trace.csv)\n")
+    print("| kernel | dispatches | average µs |")
+    print("|---|---:|---:|")
+    for name, ds in sorted(by.items(), key=lambda kv: -sum(kv[1][-last:])):
+        if name.startswith("__amd") or len(ds) < last:
+            continue
+        t = ds[-last:]
+        print(f"| `{name}` | {len(t)} | {sum(t) / len(t):.2f} |")
 
 
 def main():
@@ -782,14 +802,15 @@ def main():
     ap.add_argument("--summarize-counters", nargs=3, metavar=("WORKLOAD", "DIR", "OUT_JSON"),
                     help="summarise rocprofv3 --pmc runs under DIR (subfolders pmc_* and the copy calibration "
                          "cal_fetch, cal_write) into per-launch figures by stage; prints a markdown table")
-    ap.add_argument("--summarize-trace", nargs=1, metavar="DIR",
-                    help="print the rocprofv3 --stats kernel summary under DIR as a markdown table")
+    ap.add_argument("--summarize-trace", nargs="+", metavar="DIR [LAST]",
+                    help="print the rocprofv3 --stats kernel summary under DIR as a markdown table, and with LAST "
+                         "each kernel's mean over its last LAST dispatches")
     args = ap.parse_args()
     if args.summarize_counters:
         summarize_counters(*args.summarize_counters)
         return
     if args.summarize_trace:
-        summarize_trace(args.summarize_trace[0])
+        summarize_trace(args.summarize_trace[0], int(args.summarize_trace[1]) if len(args.summarize_trace) > 1 else 0)
         return
     if args.copy_calibration:
         import torch
