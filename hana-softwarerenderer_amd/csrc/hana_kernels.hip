@@ -1209,8 +1209,14 @@ __device__ __forceinline__ void shade_pixel(const PassDesc& P, const Workspace& 
     }
     TriS t;
     tri_setup(va, vb, vc, P.width, P.height, t);
+    // The weights take the same quotients as the resolve.  Where a numerator
+    // is zero, tri_cover_fast may give a zero weight of the other sign than the
+    // division: wx = 1 - q is then 1 either way, and a zero wy or wz only adds
+    // a signed zero beside the nonzero va.w * wx in the normaliser and in each
+    // varying's sum, so no sum changes unless it is exactly zero, and then only
+    // in the sign of that zero.  Depth is summed from +0 and does not change.
     float wx, wy, wz, z;
-    tri_cover(t, px, py, wx, wy, wz, z);
+    tri_cover_fast(t, px, py, wx, wy, wz, z);
     float k0 = va.w * wx, k1 = vb.w * wy, k2 = vc.w * wz;
     float normalizer = 1 / (k0 + k1 + k2);
     float vy[8];
