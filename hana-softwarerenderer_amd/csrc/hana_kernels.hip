@@ -107,25 +107,6 @@ __device__ __forceinline__ bool tri_setup(float4 a, float4 b, float4 c, int W, i
     return t.x1 >= t.x0 && t.y1 >= t.y0;
 }
 
-// barycentric + coverage + interpolate_depth at integer pixel (px, py)
-// (SR/graphics.cpp:186-194, 222-233, 352-356).
-__device__ __forceinline__ bool tri_cover(const TriS& t, int px, int py, float& wx, float& wy, float& wz, float& z) {
-    float Px = (float)px, Py = (float)py;
-    float s0z = t.ax - Px;
-    float s1z = t.ay - Py;
-    float ux = t.s0y * s1z - s0z * t.s1y;
-    float uy = s0z * t.s1x - t.s0x * s1z;
-    wx = 1.f - (ux + uy) / t.uz;
-    wy = uy / t.uz;
-    wz = ux / t.uz;
-    if (wx < 0 || wy < 0 || wz < 0) return false;
-    z = 0.0f;
-    z += t.z2 * wz;
-    z += t.z1 * wy;
-    z += t.z0 * wx;
-    return true;
-}
-
 // n / u.z as the hardware's IEEE division sequence computes it when its
 // scaling steps are no-ops, from the reciprocal r = cover_recip(u.z) that
 // sequence refines (the same operations in the same order, so the same bits):
@@ -144,16 +125,18 @@ __device__ __forceinline__ float cover_quot(float n, float uz, float r) {
     return fmaf(e, r, q);
 }
 
-// tri_cover with the three divisions from one refined reciprocal.  The
-// division sequence (div_scale, rcp, four fma, div_fmas, div_fixup) scales its
-// operands only when the quotient, the reciprocal or the numerator is near
-// the end of the exponent range, and its fixup changes the result only for
-// zero, infinite or NaN operands or a quotient out of range.  With the
-// triangle's screen coordinates and edges within 2^22 and pixels within 2^20,
-// |n| <= 2^47 and 2^-7 < |u.z| <= 2^45, so for every numerator with |n| >=
-// 2^-80 neither happens and cover_quot gives the division's bits.  A zero
-// numerator gives a zero of either sign, which changes neither the coverage
-// test nor the depth (summed from +0).  Other candidates divide.
+// barycentric + coverage + interpolate_depth at integer pixel (px, py)
+// (SR/graphics.cpp:186-194, 222-233, 352-356), the three divisions by u.z
+// from one refined reciprocal.  The division sequence (div_scale, rcp, four
+// fma, div_fmas, div_fixup) scales its operands only when the quotient, the
+// reciprocal or the numerator is near the end of the exponent range, and its
+// fixup changes the result only for zero, infinite or NaN operands or a
+// quotient out of range.  With the triangle's screen coordinates and edges
+// within 2^22 and pixels within 2^20, |n| <= 2^47 and 2^-7 < |u.z| <= 2^45,
+// so for every numerator with |n| >= 2^-80 neither happens and cover_quot
+// gives the division's bits.  A zero numerator gives a zero of either sign,
+// which changes neither the coverage test nor the depth (summed from +0).
+// Other candidates divide.
 __device__ __forceinline__ bool tri_cover_fast(const TriS& t, int px, int py, float& wx, float& wy, float& wz, float& z) {
     float Px = (float)px, Py = (float)py;
     float s0z = t.ax - Px;
