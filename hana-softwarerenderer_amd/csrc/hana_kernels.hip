@@ -147,10 +147,22 @@ __device__ __forceinline__ bool tri_cover_fast(const TriS& t, int px, int py, fl
     float ext = fmaxf(fmaxf(fmaxf(fabsf(t.ax), fabsf(t.ay)), fmaxf(fabsf(t.s0x), fabsf(t.s0y))),
                       fmaxf(fabsf(t.s1x), fabsf(t.s1y)));
     float r = cover_recip(t.uz);
-    // zero, or at least 2^-80 in magnitude: (|bits| << 1) - 1 >= (bits(2^-80) << 1) - 1
-    uint32_t m = min(min((__float_as_uint(sum) << 1) - 1u, (__float_as_uint(uy) << 1) - 1u),
-                     (__float_as_uint(ux) << 1) - 1u);
-    if (ext <= 0x1p22f && m >= ((uint32_t)(127 - 80) << 24) - 1u) {
+    // The shortcut needs every nonzero numerator to be at least 2^-80 in
+    // magnitude.  When every input of the triangle is zero or at least 2^-8 in
+    // magnitude (and at most 2^22), a nonzero numerator is at least 2^-62:
+    // ax and ay are multiples of 2^-31, so s0z and s1z are (their rounding
+    // keeps that); a product of a nonzero edge component and a nonzero s0z or
+    // s1z is at least 2^-39, so a multiple of 2^-62; their differences and
+    // the sum stay multiples of 2^-62.  Such a triangle needs no check per
+    // candidate; the others (an input under 2^-8 but not zero: rare) divide.
+    // v is zero or at least 2^-8 in magnitude when (bits(v) << 1) - 1 >=
+    // (bits(2^-8) << 1) - 1 (zero wraps to the maximum).
+    const uint32_t kMin8 = ((uint32_t)(127 - 8) << 24) - 1u;
+    uint32_t mt = min(min(min((__float_as_uint(t.ax) << 1) - 1u, (__float_as_uint(t.ay) << 1) - 1u),
+                          min((__float_as_uint(t.s0x) << 1) - 1u, (__float_as_uint(t.s0y) << 1) - 1u)),
+                      min((__float_as_uint(t.s1x) << 1) - 1u, (__float_as_uint(t.s1y) << 1) - 1u));
+    bool fast = ext <= 0x1p22f && mt >= kMin8;
+    if (fast) {
         wx = 1.f - cover_quot(sum, t.uz, r);
         wy = cover_quot(uy, t.uz, r);
         wz = cover_quot(ux, t.uz, r);

@@ -137,3 +137,85 @@ def test_byte_over_255_is_the_division(tmp_path):
     subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-o", str(exe), str(c), "-lm"], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
     assert out.returncode == 0 and "bad 0" in out.stdout, out.stdout
+
+
+LEMMA_SRC = r"""
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+static uint64_t st = 0xD1B54A32D192ED03ull;
+static uint64_t nxt(void) {   /* splitmix64 */
+    uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static float bits(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+static uint32_t ubits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+/* an input the per-triangle test admits: zero, or 2^-8 <= |v| <= 2^22, drawn
+   near the ends of that range, near integers and near 1 - 2^-24 */
+static float input(void) {
+    uint64_t r = nxt();
+    uint32_t s = (uint32_t)(r >> 63) << 31, m = (uint32_t)(r >> 20) & 0x7FFFFFu;
+    switch (r % 7) {
+    case 0: return 0.0f;
+    case 1: return bits(s | ((uint32_t)(127 - 8) << 23) | m);              /* [2^-8, 2^-7) */
+    case 2: return bits(s | ((uint32_t)(127 - 8) << 23) | (m & 0xF));      /* just above 2^-8 */
+    case 3: return bits(s | ((uint32_t)(127 + 21) << 23) | m);             /* [2^21, 2^22) */
+    case 4: { float k = (float)(int32_t)(nxt() % 1000); int d = (int)(nxt() % 9) - 4;
+              float v = k; for (int i = 0; i < abs(d); i++) v = nextafterf(v, d < 0 ? -INFINITY : INFINITY);
+              return fabsf(v) >= 0x1p-8f ? v : 0.0f; }                    /* integers and their neighbours */
+    case 5: return (r & 1) ? 1.0f - 0x1p-24f : -(1.0f - 0x1p-24f);
+    default: { int e = (int)(nxt() % 30) - 8; return bits(s | ((uint32_t)(127 + e) << 23) | m); }
+    }
+}
+int main(int argc, char** argv) {
+    long n = argc > 1 ? atol(argv[1]) : 1000000, bad = 0, nonzero = 0;
+    float least = INFINITY;
+    for (long i = 0; i < n; i++) {
+        float ax = input(), ay = input(), s0x = input(), s0y = input(), s1x = input(), s1y = input();
+        if (fabsf(ax) > 0x1p22f || fabsf(ay) > 0x1p22f) continue;
+        for (int k = 0; k < 8; k++) {
+            /* pixels at and next to the vertex, and anywhere */
+            long px = k < 4 ? lroundf(ax) + (long)(nxt() % 3) - 1 : (long)(nxt() % (1u << 20));
+            long py = k < 4 ? lroundf(ay) + (long)(nxt() % 3) - 1 : (long)(nxt() % (1u << 20));
+            if (px < 0 || py < 0 || px > (1 << 20) || py > (1 << 20)) continue;
+            float Px = (float)px, Py = (float)py;
+            float s0z = ax - Px, s1z = ay - Py;
+            float p1 = s0y * s1z, p2 = s0z * s1y, p3 = s0z * s1x, p4 = s0x * s1z;
+            float ux = p1 - p2, uy = p3 - p4, sum = ux + uy;
+            float v[3] = {ux, uy, sum};
+            for (int j = 0; j < 3; j++) {
+                if (v[j] == 0.0f) continue;
+                nonzero++;
+                if (fabsf(v[j]) < least) least = fabsf(v[j]);
+                if (fabsf(v[j]) < 0x1p-62f) {
+                    if (bad < 5) printf("SMALL %a from %a %a %a %a %a %a at %ld %ld\n", v[j], ax, ay, s0x, s0y, s1x, s1y, px, py);
+                    bad++;
+                }
+            }
+        }
+    }
+    printf("nonzero %ld least %a bad %ld\n", nonzero, least, bad);
+    return bad != 0;
+}
+"""
+
+
+def test_numerators_of_admitted_triangles_are_not_tiny(tmp_path):
+    """The raster skips the per-candidate range check for a triangle whose
+    inputs (a's screen position and both edges) are each zero or at least 2^-8
+    in magnitude: then every nonzero numerator u.x, u.y, u.x + u.y at an integer
+    pixel is at least 2^-62, inside the range where the shared-reciprocal
+    quotient equals the division (the test above).  Adversarial inputs: the
+    ends of the admitted range, integers and their neighbours, 1 - 2^-24, and
+    pixels at and next to the vertex."""
+    c = tmp_path / "lemma.c"
+    c.write_text(LEMMA_SRC)
+    exe = tmp_path / "lemma"
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-o", str(exe), str(c), "-lm"], check=True)
+    out = subprocess.run([str(exe), "2000000"], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0 and "bad 0" in out.stdout, out.stdout[-2000:]
