@@ -86,3 +86,79 @@ def empty_mesh():
 def faceless(mesh):
     """The vertices of mesh with no faces."""
     return sg.Mesh(mesh.pos, mesh.nrm, mesh.uv, np.zeros((0, 3), np.int32))
+
+
+def tiny_input_scene(n=4000, width=256, height=192, seed=21):
+    """Triangles whose screen-space set-up has tiny inputs, beside ordinary
+    ones: a corner within 2^-8 px of the left or top edge of the target, edges
+    a few ulps from vertical or horizontal (a screen component of 2^-16 px or
+    so), and corners exactly at 2^-8 px from the edge and one ulp inside it.
+    The raster's coverage takes its shared-reciprocal shortcut only for
+    triangles whose inputs are each zero or at least 2^-8 px, and divides for
+    the others, so both paths and their boundary are drawn.  Sizes run from a
+    pixel to a third of the target, so the per-lane, shared and wave-wide
+    sweeps all see them.  With camera_vp the identity, positions are NDC."""
+    rng = np.random.default_rng(seed)
+    f32 = np.float32
+    W, H = f32(width), f32(height)
+
+    def ndc_x(sx):
+        return f32(f32(sx) / W * f32(2) - f32(1))
+
+    def ndc_y(sy):
+        return f32(f32(sy) / H * f32(2) - f32(1))
+
+    tris = []
+    for i in range(n):
+        kind = i % 6
+        size = float(np.exp(rng.uniform(np.log(1.0), np.log(min(width, height) / 3))))
+        ax, ay = rng.uniform(0, width - 1), rng.uniform(0, height - 1)
+        a = [ndc_x(ax), ndc_y(ay)]
+        b = [ndc_x(ax + rng.uniform(-1, 1) * size), ndc_y(ay + rng.uniform(-1, 1) * size)]
+        c = [ndc_x(ax + rng.uniform(-1, 1) * size), ndc_y(ay + rng.uniform(-1, 1) * size)]
+        if kind == 0:      # a corner within 2^-8 px of the left edge: ndc -1 + k 2^-24
+            a[0] = f32(-1) + f32(int(rng.integers(1, 400))) * f32(2.0 ** -24)
+            b[0] = ndc_x(rng.uniform(0.5, size + 1))
+            c[0] = ndc_x(rng.uniform(0.5, size + 1))
+        elif kind == 1:    # the same at the top edge
+            a[1] = f32(-1) + f32(int(rng.integers(1, 400))) * f32(2.0 ** -24)
+            b[1] = ndc_y(rng.uniform(0.5, size + 1))
+            c[1] = ndc_y(rng.uniform(0.5, size + 1))
+        elif kind == 2:    # edge a-b a few ulps from vertical
+            b[0] = a[0]
+            for _ in range(int(rng.integers(1, 4))):
+                b[0] = np.nextafter(b[0], f32(np.inf) if rng.uniform() < 0.5 else f32(-np.inf))
+        elif kind == 3:    # edge a-c a few ulps from horizontal
+            c[1] = a[1]
+            for _ in range(int(rng.integers(1, 4))):
+                c[1] = np.nextafter(c[1], f32(np.inf) if rng.uniform() < 0.5 else f32(-np.inf))
+        elif kind == 4:    # a corner at 2^-8 px from the left edge, or one ulp inside
+            a[0] = f32(-1) + f32(2.0 ** -15) - (f32(2.0 ** -24) if rng.uniform() < 0.5 else f32(0))
+            b[0] = ndc_x(rng.uniform(0.5, size + 1))
+            c[0] = ndc_x(rng.uniform(0.5, size + 1))
+        z = rng.uniform(-0.9, 0.9, 3).astype(f32)
+        p = np.array([[a[0], a[1], z[0]], [b[0], b[1], z[1]], [c[0], c[1], z[2]]], f32)
+        area = (float(p[1, 0]) - float(p[0, 0])) * (float(p[2, 1]) - float(p[0, 1])) - \
+               (float(p[2, 0]) - float(p[0, 0])) * (float(p[1, 1]) - float(p[0, 1]))
+        if area < 0:
+            p = p[[0, 2, 1]]
+        tris.append(p)
+    pos = np.concatenate(tris).astype(f32)
+    uv = rng.uniform(0, 1, (3 * n, 2)).astype(f32)
+    nrm = np.tile(np.array([[0.0, 0.0, 1.0]], f32), (3 * n, 1))
+    mesh = sg.Mesh(pos, nrm, uv, np.arange(3 * n, dtype=np.int32).reshape(n, 3))
+    base = sg.small_scene(sg.TEXTURE, width=64, height=64, seed=seed)
+    return sg.Scene(mesh, width, height, sg.TEXTURE, False, (0.0, 0.0, 2.0), base.light_pos, base.diffuse,
+                    base.normal, name="tiny inputs")
+
+
+def screen_inputs(scene):
+    """Per face, the six screen-space set-up inputs (a's position, both edges)
+    as the raster forms them from NDC corners, in float32."""
+    f32 = np.float32
+    p = scene.mesh.pos[scene.mesh.idx]           # (F, 3, 3) NDC with camera_vp the identity
+    W, H = f32(scene.width), f32(scene.height)
+    sx = (p[..., 0] + f32(1)) * f32(0.5) * W
+    sy = (p[..., 1] + f32(1)) * f32(0.5) * H
+    return np.stack([sx[:, 0], sy[:, 0], sx[:, 2] - sx[:, 0], sx[:, 1] - sx[:, 0],
+                     sy[:, 2] - sy[:, 0], sy[:, 1] - sy[:, 0]], 1).astype(f32)

@@ -7,7 +7,8 @@ interpolated light-space depth) bit-exact.
 import numpy as np
 import pytest
 
-from edge_scenes import IDENTITY_VP, empty_mesh, faceless, nan_depth_scene, sliver_scene, with_camera_vp
+from edge_scenes import (IDENTITY_VP, empty_mesh, faceless, nan_depth_scene, screen_inputs, sliver_scene,
+                         tiny_input_scene, with_camera_vp)
 from hana_amd import scenegen as sg
 from parity import assert_color_ok, assert_depth_exact
 from pyoracle import CPURenderer, HostTarget
@@ -265,6 +266,31 @@ def test_widest_target(ctx, oracle):
     assert np.array_equal(win.astype(np.int64), as_u32_ids(owin))
     assert_depth_exact(dep, ref.depth, "widest")
     assert_color_ok(col, ref.color, "widest")
+
+
+def test_tiny_screen_inputs(ctx, oracle):
+    """Coverage of set-ups with tiny inputs: corners within 2^-8 px of the
+    target's edge and edges a few ulps from vertical or horizontal (their
+    triangles take the raster's division path), and corners exactly 2^-8 px
+    from the edge (the shared-reciprocal path at its boundary).  Winners,
+    depth and colour equal the oracle's."""
+    from hana_amd.gpu import build_uniforms
+    sc = tiny_input_scene()
+    a = np.abs(screen_inputs(sc))
+    assert ((a > 0) & (a < 2.0 ** -8)).any(1).sum() > 1000 and (a == 2.0 ** -8).any(1).sum() > 100
+    W, H = sc.width, sc.height
+    tgt = ctx.target(W, H, winner=True)
+    u = with_camera_vp(build_uniforms(sc), IDENTITY_VP)
+    ctx.draw(ctx.mesh(sc.mesh), sc.shader, u, tgt, ctx.texture(sc.diffuse), ctx.texture(sc.normal))
+    col, dep = ctx.read(tgt, W, H)
+    win = ctx.read_winner(tgt, W, H)
+    ref = HostTarget(W, H)
+    st, owin = oracle.draw(sc.mesh, sc.shader, with_camera_vp(oracle.build_uniforms(sc), IDENTITY_VP), ref,
+                           sc.diffuse, sc.normal, winner=True)
+    assert st["fragments"] > W * H
+    assert np.array_equal(win.astype(np.int64), as_u32_ids(owin)), "winners"
+    assert_depth_exact(dep, ref.depth, "tiny inputs")
+    assert_color_ok(col, ref.color, "tiny inputs")
 
 
 def test_pass_statistics(ctx, oracle):

@@ -12,7 +12,7 @@ import os
 import numpy as np
 import pytest
 
-from edge_scenes import nan_depth_scene, with_camera_vp
+from edge_scenes import IDENTITY_VP, nan_depth_scene, tiny_input_scene, with_camera_vp
 from hana_amd import scenegen as sg
 from pyoracle import REF_SO, CPURenderer, HostTarget, Uniforms
 
@@ -178,6 +178,21 @@ def test_live_nan_depth_matches_reference(oracle, ref):
     assert np.isnan(out[0].depth).sum() > 300
     assert beq(out[0].depth, out[1].depth)
     assert np.array_equal(out[0].color[..., :3], out[1].color[..., :3])
+
+
+def test_live_tiny_inputs_match_reference(oracle, ref):
+    """Set-ups with tiny inputs (corners within 2^-8 px of the target's edge,
+    edges a few ulps from vertical or horizontal): the oracle's depth, colour
+    and fragment count equal the reference's."""
+    sc = tiny_input_scene()
+    out = []
+    for r in (oracle, ref):
+        t = HostTarget(sc.width, sc.height)
+        st, _ = r.draw(sc.mesh, sc.shader, with_camera_vp(r.build_uniforms(sc), IDENTITY_VP), t, sc.diffuse, sc.normal)
+        out.append((t, st))
+    assert out[0][1]["fragments"] == out[1][1]["fragments"] > sc.width * sc.height
+    assert beq(out[0][0].depth, out[1][0].depth)
+    assert np.array_equal(out[0][0].color[..., :3], out[1][0].color[..., :3])
 
 
 def test_live_config1_matches_reference(oracle, ref):
