@@ -1336,7 +1336,8 @@ static int batch_prepare(hana_ctx* c, hana_batch* b, bool use_graph) {
         // (a run to warm up, then a timed one: the sequence clears what it
         // draws, so the runs leave nothing behind), its frames' estimates are
         // scaled by its measured time per unit of estimate, and the frames
-        // are split again; if any part's frames changed, every part is sized again.
+        // are split again (unless a part measured no time); if any part's frames
+        // changed, every part is sized again.
         std::vector<double> cost(w.size(), 0.0);
         hipEvent_t e0 = nullptr, e1 = nullptr;
         HIP_TRY(c, hipEventCreate(&e0));
@@ -1345,6 +1346,7 @@ static int batch_prepare(hana_ctx* c, hana_batch* b, bool use_graph) {
             return fail(c, HANA_E_DEVICE, "hipEventCreate");
         }
         r = HANA_OK;
+        bool measured = true;   // every part took a positive time
         for (auto& part : b->parts) {
             float ms = 0.0f;
             for (int rep = 0; rep < 2 && !r; rep++) {
@@ -1356,6 +1358,7 @@ static int batch_prepare(hana_ctx* c, hana_batch* b, bool use_graph) {
             }
             double W = 0.0;
             for (size_t k = part.f0; k < part.f1; k++) W += w[k];
+            measured = measured && ms > 0.0f && W > 0.0;
             for (size_t k = part.f0; k < part.f1; k++) cost[k] = W > 0.0 ? w[k] * (double)ms / W : w[k];
         }
         (void)hipEventDestroy(e0);
@@ -1364,7 +1367,7 @@ static int batch_prepare(hana_ctx* c, hana_batch* b, bool use_graph) {
         if (r) return r;
         std::vector<size_t> ends = split_ends(b, cost), before;
         for (auto& part : b->parts) before.push_back(part.f1);
-        if (ends != before) {
+        if (measured && ends != before) {
             batch_split(b, ends);
             for (auto& part : b->parts)
                 if ((r = batch_size_part(c, b, part))) return r;
